@@ -1,0 +1,552 @@
+// C ABI of libflurry_hip (declared in include/flurry_hip.h).
+#include <hip/hip_runtime.h>
+
+#include <cmath>
+#include <cstring>
+#include <stdexcept>
+#include <string>
+#include <vector>
+
+#include "fhip_internal.hpp"
+
+namespace {
+
+thread_local std::string g_err;
+
+struct ApiError : std::runtime_error {
+  int code;
+  ApiError(int c, const std::string& m) : std::runtime_error(m), code(c) {}
+};
+
+#define HCHK(x)                                                                           \
+  do {                                                                                    \
+    hipError_t e_ = (x);                                                                  \
+    if (e_ != hipSuccess) throw ApiError(FHIP_EDEVICE, std::string("HIP: ") + hipGetErrorString(e_)); \
+  } while (0)
+
+template <class F>
+int guard(F&& f) {
+  try {
+    return f();
+  } catch (const ApiError& e) {
+    g_err = e.what();
+    return e.code;
+  } catch (const std::invalid_argument& e) {
+    g_err = e.what();
+    return FHIP_EINVAL;
+  } catch (const std::exception& e) {
+    g_err = e.what();
+    return FHIP_EDEVICE;
+  }
+}
+
+void require(bool ok, const std::string& msg) {
+  if (!ok) throw ApiError(FHIP_EINVAL, msg);
+}
+
+template <class T>
+T* dalloc(size_t n) {
+  void* p = nullptr;
+  if (n == 0) n = 1;
+  HCHK(hipMalloc(&p, n * sizeof(T)));
+  return (T*)p;
+}
+
+template <class T>
+T* dcopy(const T* h, size_t n, hipStream_t s) {
+  T* d = dalloc<T>(n);
+  if (h && n) HCHK(hipMemcpyAsync(d, h, n * sizeof(T), hipMemcpyHostToDevice, s));
+  else if (n) HCHK(hipMemsetAsync(d, 0, n * sizeof(T), s));
+  return d;
+}
+
+// Right-element flux point of left flux point i (src/intFace.cpp:57-95).
+int right_fpt(int D, int P, int Ff, int idr, int relRot, int i) {
+  if (D == 2) return idr * Ff + Ff - 1 - i;
+  int P1 = P + 1, s = Ff * idr, e = Ff * (idr + 1), ii = i % P1, jj = i / P1;
+  switch (relRot) {
+    case 0: return s + ii * P1 + jj;
+    case 1: return s + P - ii + jj * P1;
+    case 2: return e - 1 - (ii * P1 + jj);
+    default: return e - P1 * (jj + 1) + ii;
+  }
+}
+
+void destroy_ctx(fhip::Ctx& c) {
+  if (c.device >= 0) (void)hipSetDevice(c.device);
+  for (auto& row : c.graph)
+    for (auto& g : row)
+      if (g) (void)hipGraphExecDestroy(g);
+  void* ptrs[] = {c.sf, c.grad, c.efn, c.corr, c.corru, c.ws, c.wf, c.tp, c.jginv, c.detj, c.norm, c.da,
+                  c.ft.ptL, c.ft.ptR, c.ft.bc, c.U, c.U0, c.Uf, c.Fs, c.dU, c.dUf, c.dUc, c.disFn, c.Fn,
+                  c.divF, c.waveSp, c.eleDt, c.faceFn, c.src, c.Uacc, c.d_dt, c.partial, c.flag};
+  for (void* p : ptrs)
+    if (p) (void)hipFree(p);
+  if (c.ev0) (void)hipEventDestroy(c.ev0);
+  if (c.ev1) (void)hipEventDestroy(c.ev1);
+  if (c.stream) (void)hipStreamDestroy(c.stream);
+}
+
+double* array_ptr(fhip::Ctx& c, int which, int step, long long* n) {
+  const fhip::Sizes& z = c.z;
+  switch (which) {
+    case FHIP_ARR_U_SPTS: *n = z.nS(); return c.U;
+    case FHIP_ARR_U0: *n = z.nS(); return c.U0;
+    case FHIP_ARR_U_FPTS: *n = z.nF(); return c.Uf;
+    case FHIP_ARR_F_SPTS: *n = z.D * z.nS(); return c.Fs;
+    case FHIP_ARR_DIVF_SPTS:
+      require(step >= 0 && step < z.nRK, "RK step out of range");
+      *n = z.nS();
+      return c.divF + step * z.nS();
+    case FHIP_ARR_DISFN_FPTS: *n = z.nF(); return c.disFn;
+    case FHIP_ARR_FN_FPTS: *n = z.nF(); return c.Fn;
+    case FHIP_ARR_DU_SPTS: *n = c.dU ? z.D * z.nS() : 0; return c.dU;
+    case FHIP_ARR_DU_FPTS: *n = c.dUf ? z.D * z.nF() : 0; return c.dUf;
+    case FHIP_ARR_DUC_FPTS: *n = c.dUc ? z.nF() : 0; return c.dUc;
+    case FHIP_ARR_WAVESP: *n = (long long)z.F * z.E; return c.waveSp;
+    case FHIP_ARR_ELE_DT: *n = z.E; return c.eleDt;
+    default: throw ApiError(FHIP_EINVAL, "unknown array id");
+  }
+}
+
+}  // namespace
+
+extern "C" {
+
+const char* fhip_last_error(void) { return g_err.c_str(); }
+
+int fhip_set_device(int device) {
+  return guard([&] {
+    HCHK(hipSetDevice(device));
+    return 0;
+  });
+}
+
+int fhip_create(const fhip_desc* d, fhip_ctx** out) {
+  return guard([&] {
+    require(d && d->params && out, "null descriptor");
+    const fhip_params& p = *d->params;
+    require(p.equation == 1, "only the Euler/Navier-Stokes equations (equation 1) are supported");
+    require(p.n_dims == 2 || p.n_dims == 3, "nDims must be 2 or 3");
+    require(p.n_fields == p.n_dims + 2, "nFields must be nDims+2");
+    require(p.order >= 0 && p.order <= 10, "order out of range");
+    int P1 = p.order + 1;
+    int Ff = (p.n_dims == 2) ? P1 : P1 * P1;
+    require(p.n_fpts_face == Ff && p.n_spts == (p.n_dims == 2 ? P1 * P1 : P1 * P1 * P1) && p.n_fpts == 2 * p.n_dims * Ff,
+            "point counts do not match nDims and order");
+    require(p.n_eles > 0, "nEles must be positive");
+    require(p.n_rk_steps >= 1 && p.n_rk_steps <= 7, "nRKSteps out of range");
+    require(!(p.n_dims == 3 && p.riemann_type == 1), "Roe not implemented in 3D");
+    require(p.riemann_type == 0 || p.riemann_type == 1, "riemannType must be 0 or 1");
+    require(p.dt_type >= 0 && p.dt_type <= 2, "dtType must be 0, 1 or 2");
+    require((long long)p.n_fpts * p.n_eles < (1ll << 31), "mesh too large for 32-bit point indices");
+    require(d->opp_spts_to_fpts && d->opp_grad_spts && d->opp_extrapolate_fn && d->opp_correction && d->w_spts &&
+                d->w_face && d->jginv_spts && d->detjac_spts && d->norm_fpts && d->da_fpts,
+            "missing operator or metric array");
+    require(!p.viscous || d->opp_correct_u, "viscous runs need opp_correct_u");
+    require(d->n_faces >= 0 && (d->n_faces == 0 || (d->face_ele_l && d->face_loc_l && d->face_ele_r && d->face_loc_r &&
+                                                   d->face_rel_rot && d->face_bc)),
+            "missing face arrays");
+
+    auto* h = new fhip_ctx();
+    fhip::Ctx& c = h->c;
+    try {
+      HCHK(hipGetDevice(&c.device));
+      c.prm = p;
+      fhip::Sizes& z = c.z;
+      z.D = p.n_dims; z.N = p.n_fields; z.P = p.order; z.S = p.n_spts; z.F = p.n_fpts; z.Ff = Ff; z.E = p.n_eles;
+      z.nRK = p.n_rk_steps; z.nFaces = d->n_faces; z.nFaceFpts = d->n_faces * Ff;
+      c.ph = fhip::Phys{p.gamma, p.prandtl, p.mu_inf, p.rt_inf, p.c_sth, p.pen_fact, p.tau,
+                        p.rho_bound, p.u_bound, p.v_bound, p.w_bound, p.p_bound,
+                        p.t_wall, p.r_gas, p.u_wall, p.v_wall, p.w_wall, p.fix_vis, p.riemann_type, p.viscous};
+      HCHK(hipStreamCreateWithFlags(&c.stream, hipStreamNonBlocking));
+      HCHK(hipEventCreate(&c.ev0));
+      HCHK(hipEventCreate(&c.ev1));
+      const int D = z.D, S = z.S, F = z.F, E = z.E;
+      c.sf = dcopy(d->opp_spts_to_fpts, (size_t)F * S, c.stream);
+      c.grad = dcopy(d->opp_grad_spts, (size_t)D * S * S, c.stream);
+      c.efn = dcopy(d->opp_extrapolate_fn, (size_t)D * F * S, c.stream);
+      c.corr = dcopy(d->opp_correction, (size_t)S * F, c.stream);
+      if (p.viscous) c.corru = dcopy(d->opp_correct_u, (size_t)D * S * F, c.stream);
+      c.ws = dcopy(d->w_spts, S, c.stream);
+      c.wf = dcopy(d->w_face, Ff, c.stream);
+      c.jginv = dcopy(d->jginv_spts, (size_t)D * S * E * D, c.stream);
+      c.detj = dcopy(d->detjac_spts, (size_t)S * E, c.stream);
+      c.norm = dcopy(d->norm_fpts, (size_t)F * E * D, c.stream);
+      c.da = dcopy(d->da_fpts, (size_t)F * E, c.stream);
+
+      // Face flux-point tables.
+      std::vector<int> bc((size_t)z.nFaceFpts);
+      c.h_ptL.resize(z.nFaceFpts);
+      c.h_ptR.resize(z.nFaceFpts);
+      std::vector<char> owned((size_t)F * E, 0);
+      for (int f = 0; f < z.nFaces; f++) {
+        int eL = d->face_ele_l[f], lL = d->face_loc_l[f], eR = d->face_ele_r[f];
+        require(eL >= 0 && eL < E && lL >= 0 && lL < 2 * D, "face " + std::to_string(f) + ": bad left element/face");
+        require(eR < E, "face " + std::to_string(f) + ": bad right element");
+        if (eR >= 0) {
+          int lR = d->face_loc_r[f], rr = d->face_rel_rot[f];
+          require(lR >= 0 && lR < 2 * D && (D == 2 || (rr >= 0 && rr < 4)), "face " + std::to_string(f) + ": bad right face/rotation");
+        } else {
+          int b = d->face_bc[f];
+          require(b == 1 || b == 2 || b == 3 || b == 4 || b == 5 || b == 8 || b == 9 || b == 10 || b == 12 || b == 13,
+                  "face " + std::to_string(f) + ": unsupported boundary condition " + std::to_string(b));
+        }
+        for (int i = 0; i < Ff; i++) {
+          size_t t = (size_t)f * Ff + i;
+          int fl = lL * Ff + i;
+          c.h_ptL[t] = fl * E + eL;
+          if (eR >= 0) {
+            int fr = right_fpt(D, z.P, Ff, d->face_loc_r[f], d->face_rel_rot[f], i);
+            c.h_ptR[t] = fr * E + eR;
+          } else {
+            c.h_ptR[t] = -1;
+          }
+          bc[t] = (eR >= 0) ? -1 : d->face_bc[f];
+          for (int side = 0; side < 2; side++) {
+            int pt = side ? c.h_ptR[t] : c.h_ptL[t];
+            if (pt < 0) continue;
+            require(!owned[pt], "flux point owned by two faces (face " + std::to_string(f) + ")");
+            owned[pt] = 1;
+          }
+        }
+      }
+      c.ft.ptL = dcopy(c.h_ptL.data(), c.h_ptL.size(), c.stream);
+      c.ft.ptR = dcopy(c.h_ptR.data(), c.h_ptR.size(), c.stream);
+      c.ft.bc = dcopy(bc.data(), bc.size(), c.stream);
+
+      const size_t nS = z.nS(), nF = z.nF();
+      c.U = dcopy<double>(nullptr, nS, c.stream);
+      c.U0 = dcopy<double>(nullptr, nS, c.stream);
+      c.Uf = dcopy<double>(nullptr, nF, c.stream);
+      c.Fs = dcopy<double>(nullptr, (size_t)D * nS, c.stream);
+      if (p.viscous) {
+        c.dU = dcopy<double>(nullptr, (size_t)D * nS, c.stream);
+        c.dUf = dcopy<double>(nullptr, (size_t)D * nF, c.stream);
+        c.dUc = dcopy<double>(nullptr, nF, c.stream);
+      }
+      c.disFn = dcopy<double>(nullptr, nF, c.stream);
+      c.Fn = dcopy<double>(nullptr, nF, c.stream);
+      c.divF = dcopy<double>(nullptr, (size_t)z.nRK * nS, c.stream);
+      c.waveSp = dcopy<double>(nullptr, (size_t)F * E, c.stream);
+      c.eleDt = dcopy<double>(nullptr, E, c.stream);
+      c.faceFn = dcopy<double>(nullptr, (size_t)z.nFaceFpts * z.N, c.stream);
+      c.d_dt = dcopy<double>(&p.dt, 1, c.stream);
+      c.partial = dcopy<double>(nullptr, ((size_t)E / 256 + 2) * z.N + 1024, c.stream);
+      c.flag = dcopy<int>(nullptr, 1, c.stream);
+      c.h_dt = p.dt;
+      c.iter = p.init_iter;
+      c.fused_ok = fhip::fused_supported(c);
+      if (c.fused_ok) fhip::fused_setup(c);
+      HCHK(hipStreamSynchronize(c.stream));
+    } catch (...) {
+      destroy_ctx(c);
+      delete h;
+      throw;
+    }
+    *out = h;
+    return 0;
+  });
+}
+
+void fhip_destroy(fhip_ctx* h) {
+  if (!h) return;
+  destroy_ctx(h->c);
+  delete h;
+}
+
+int fhip_set_state(fhip_ctx* h, const double* u) {
+  return guard([&] {
+    require(h && u, "null argument");
+    fhip::Ctx& c = h->c;
+    HCHK(hipMemcpyAsync(c.U, u, c.z.nS() * sizeof(double), hipMemcpyHostToDevice, c.stream));
+    HCHK(hipStreamSynchronize(c.stream));
+    return 0;
+  });
+}
+
+int fhip_get_state(fhip_ctx* h, double* u) {
+  return guard([&] {
+    require(h && u, "null argument");
+    fhip::Ctx& c = h->c;
+    HCHK(hipMemcpyAsync(u, c.U, c.z.nS() * sizeof(double), hipMemcpyDeviceToHost, c.stream));
+    HCHK(hipStreamSynchronize(c.stream));
+    return 0;
+  });
+}
+
+int fhip_extrapolate_u(fhip_ctx* h) {
+  return guard([&] {
+    require(h, "null context");
+    fhip::staged_extrapolate_u(h->c);
+    return 0;
+  });
+}
+
+int fhip_calc_wave_speed(fhip_ctx* h) {
+  return guard([&] {
+    require(h, "null context");
+    fhip::staged_wave_speed(h->c);
+    return 0;
+  });
+}
+
+int fhip_calc_residual(fhip_ctx* h, int step) {
+  return guard([&] {
+    require(h, "null context");
+    require(step >= 0 && step < h->c.z.nRK, "RK step out of range");
+    fhip::staged_residual(h->c, step);
+    return 0;
+  });
+}
+
+static void upload_src(fhip::Ctx& c, const double* src) {
+  if (!c.src) c.src = dalloc<double>(c.z.nS());
+  HCHK(hipMemcpyAsync(c.src, src, c.z.nS() * sizeof(double), hipMemcpyHostToDevice, c.stream));
+}
+
+int fhip_time_step_a(fhip_ctx* h, int step, double rk, const double* src) {
+  return guard([&] {
+    require(h, "null context");
+    require(step >= 0 && step < h->c.z.nRK, "RK step out of range");
+    if (src) upload_src(h->c, src);
+    fhip::staged_time_step_a(h->c, step, rk, src != nullptr);
+    HCHK(hipStreamSynchronize(h->c.stream));
+    return 0;
+  });
+}
+
+int fhip_time_step_b(fhip_ctx* h, int step, const double* src) {
+  return guard([&] {
+    require(h, "null context");
+    require(step >= 0 && step < h->c.z.nRK, "RK step out of range");
+    if (src) upload_src(h->c, src);
+    fhip::staged_time_step_b(h->c, step, src != nullptr);
+    HCHK(hipStreamSynchronize(h->c.stream));
+    return 0;
+  });
+}
+
+int fhip_copy_u_to_u0(fhip_ctx* h) {
+  return guard([&] {
+    require(h, "null context");
+    fhip::copy_array(h->c, h->c.U0, h->c.U, h->c.z.nS());
+    return 0;
+  });
+}
+
+int fhip_copy_u0_to_u(fhip_ctx* h) {
+  return guard([&] {
+    require(h, "null context");
+    fhip::copy_array(h->c, h->c.U, h->c.U0, h->c.z.nS());
+    return 0;
+  });
+}
+
+int fhip_calc_dt(fhip_ctx* h, int refresh, double* dt_out) {
+  return guard([&] {
+    require(h, "null context");
+    fhip::Ctx& c = h->c;
+    fhip::calc_dt(c, refresh != 0);
+    HCHK(hipMemcpyAsync(&c.h_dt, c.d_dt, sizeof(double), hipMemcpyDeviceToHost, c.stream));
+    HCHK(hipStreamSynchronize(c.stream));
+    if (dt_out) *dt_out = c.h_dt;
+    return 0;
+  });
+}
+
+int fhip_set_dt(fhip_ctx* h, double dt) {
+  return guard([&] {
+    require(h, "null context");
+    fhip::Ctx& c = h->c;
+    c.h_dt = dt;
+    HCHK(hipMemcpyAsync(c.d_dt, &c.h_dt, sizeof(double), hipMemcpyHostToDevice, c.stream));
+    HCHK(hipStreamSynchronize(c.stream));
+    return 0;
+  });
+}
+
+int fhip_get_dt(fhip_ctx* h, double* dt) {
+  return guard([&] {
+    require(h && dt, "null argument");
+    fhip::Ctx& c = h->c;
+    HCHK(hipMemcpyAsync(&c.h_dt, c.d_dt, sizeof(double), hipMemcpyDeviceToHost, c.stream));
+    HCHK(hipStreamSynchronize(c.stream));
+    *dt = c.h_dt;
+    return 0;
+  });
+}
+
+int fhip_get_iter(fhip_ctx* h, int* iter) {
+  if (!h || !iter) return FHIP_EINVAL;
+  *iter = h->c.iter;
+  return 0;
+}
+
+int fhip_update(fhip_ctx* h, int n_steps, int path) {
+  return guard([&] {
+    require(h, "null context");
+    require(n_steps >= 0, "n_steps must be >= 0");
+    require(path == FHIP_PATH_AUTO || path == FHIP_PATH_STAGED || path == FHIP_PATH_FUSED, "unknown path");
+    fhip::Ctx& c = h->c;
+    bool fused = (path == FHIP_PATH_FUSED) || (path == FHIP_PATH_AUTO && c.fused_ok);
+    require(!fused || c.fused_ok, "the fused path does not support this configuration");
+    const int nRK = c.z.nRK;
+    for (int it = 0; it < n_steps; it++) {
+      c.iter++;
+      bool refresh = (c.iter == c.prm.init_iter + 1);
+      if (fused) {
+        fhip::fused_rk_step(c, refresh);
+        continue;
+      }
+      // solver::update (src/solver.cpp:319-363), static grid
+      for (int step = 0; step < nRK - 1; step++) {
+        if (step == 0 && c.prm.dt_type != 0) fhip::calc_dt(c, refresh);
+        if (step == 0) fhip::copy_array(c, c.U0, c.U, c.z.nS());
+        fhip::staged_residual(c, step);
+        fhip::staged_time_step_a(c, step, c.prm.rk_a[step + 1], false);
+      }
+      fhip::staged_residual(c, nRK - 1);
+      if (c.prm.time_type < 5) {
+        if (nRK > 1) fhip::copy_array(c, c.U, c.U0, c.z.nS());
+        else if (c.prm.dt_type != 0) fhip::calc_dt(c, refresh);
+        for (int step = 0; step < nRK; step++) fhip::staged_time_step_b(c, step, false);
+      } else {
+        fhip::staged_time_step_a(c, nRK - 1, c.prm.rk_a[nRK], false);
+      }
+    }
+    return 0;
+  });
+}
+
+int fhip_path_supported(fhip_ctx* h, int path) {
+  if (!h) return 0;
+  if (path == FHIP_PATH_STAGED || path == FHIP_PATH_AUTO) return 1;
+  if (path == FHIP_PATH_FUSED) return h->c.fused_ok ? 1 : 0;
+  return 0;
+}
+
+int fhip_sync(fhip_ctx* h) {
+  return guard([&] {
+    require(h, "null context");
+    HCHK(hipStreamSynchronize(h->c.stream));
+    return 0;
+  });
+}
+
+int fhip_residual_norm(fhip_ctx* h, int res_type, double* out) {
+  return guard([&] {
+    require(h && out, "null argument");
+    require(res_type >= 1 && res_type <= 3, "resType must be 1, 2 or 3");
+    fhip::residual_norm(h->c, res_type, out);
+    return 0;
+  });
+}
+
+int fhip_check_state(fhip_ctx* h) {
+  return guard([&] {
+    require(h, "null context");
+    if (fhip::check_state(h->c)) throw ApiError(FHIP_ESTATE, "non-finite or non-physical state (NaN/Inf, rho<=0 or p<=0)");
+    return 0;
+  });
+}
+
+long long fhip_array_size(fhip_ctx* h, int which) {
+  if (!h) return -1;
+  if (which == FHIP_ARR_FACE_FPT_L || which == FHIP_ARR_FACE_FPT_R) return h->c.z.nFaceFpts;
+  long long n = -1;
+  int rc = guard([&] {
+    array_ptr(h->c, which, 0, &n);
+    return 0;
+  });
+  return rc ? -1 : n;
+}
+
+int fhip_get_array(fhip_ctx* h, int which, int step, void* host) {
+  return guard([&] {
+    require(h && host, "null argument");
+    fhip::Ctx& c = h->c;
+    if (which == FHIP_ARR_FACE_FPT_L || which == FHIP_ARR_FACE_FPT_R) {
+      const std::vector<int>& v = (which == FHIP_ARR_FACE_FPT_L) ? c.h_ptL : c.h_ptR;
+      std::memcpy(host, v.data(), v.size() * sizeof(int));
+      return 0;
+    }
+    long long n;
+    double* d = array_ptr(c, which, step, &n);
+    require(d != nullptr && n > 0, "array not allocated in this configuration");
+    HCHK(hipStreamSynchronize(c.stream));
+    if (which == FHIP_ARR_WAVESP) {
+      // device [fpt][ele] -> reference per-element vectors [ele][fpt]
+      std::vector<double> tmp(n);
+      HCHK(hipMemcpy(tmp.data(), d, n * sizeof(double), hipMemcpyDeviceToHost));
+      double* o = (double*)host;
+      for (int f = 0; f < c.z.F; f++)
+        for (int e = 0; e < c.z.E; e++) o[(size_t)e * c.z.F + f] = tmp[(size_t)f * c.z.E + e];
+      return 0;
+    }
+    HCHK(hipMemcpy(host, d, n * sizeof(double), hipMemcpyDeviceToHost));
+    return 0;
+  });
+}
+
+int fhip_set_array(fhip_ctx* h, int which, int step, const void* host) {
+  return guard([&] {
+    require(h && host, "null argument");
+    fhip::Ctx& c = h->c;
+    require(which != FHIP_ARR_FACE_FPT_L && which != FHIP_ARR_FACE_FPT_R, "face maps are read-only");
+    long long n;
+    double* d = array_ptr(c, which, step, &n);
+    require(d != nullptr && n > 0, "array not allocated in this configuration");
+    HCHK(hipStreamSynchronize(c.stream));
+    if (which == FHIP_ARR_WAVESP) {
+      std::vector<double> tmp(n);
+      const double* in = (const double*)host;
+      for (int f = 0; f < c.z.F; f++)
+        for (int e = 0; e < c.z.E; e++) tmp[(size_t)f * c.z.E + e] = in[(size_t)e * c.z.F + f];
+      HCHK(hipMemcpy(d, tmp.data(), n * sizeof(double), hipMemcpyHostToDevice));
+      return 0;
+    }
+    HCHK(hipMemcpy(d, host, n * sizeof(double), hipMemcpyHostToDevice));
+    return 0;
+  });
+}
+
+int fhip_device_ptr(fhip_ctx* h, int which, int step, void** dev) {
+  return guard([&] {
+    require(h && dev, "null argument");
+    long long n;
+    *dev = array_ptr(h->c, which, step, &n);
+    return 0;
+  });
+}
+
+int fhip_stream(fhip_ctx* h, void** stream) {
+  if (!h || !stream) return FHIP_EINVAL;
+  *stream = (void*)h->c.stream;
+  return 0;
+}
+
+int fhip_enable_timing(fhip_ctx* h, int on) {
+  if (!h) return FHIP_EINVAL;
+  h->c.timing = on != 0;
+  h->c.timer.acc.clear();
+  return 0;
+}
+
+int fhip_kernel_times(fhip_ctx* h, int max, char* names, int name_len, double* avg_ms, long long* counts) {
+  if (!h) return -1;
+  int i = 0;
+  for (auto& kv : h->c.timer.acc) {
+    if (i >= max) break;
+    if (names) {
+      std::strncpy(names + (size_t)i * name_len, kv.first.c_str(), name_len - 1);
+      names[(size_t)i * name_len + name_len - 1] = 0;
+    }
+    if (avg_ms) avg_ms[i] = kv.second.second ? kv.second.first / kv.second.second : 0;
+    if (counts) counts[i] = kv.second.second;
+    i++;
+  }
+  return i;
+}
+
+}  // extern "C"

