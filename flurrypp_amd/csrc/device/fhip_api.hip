@@ -74,6 +74,7 @@ int right_fpt(int D, int P, int Ff, int idr, int relRot, int i) {
 
 void destroy_ctx(fhip::Ctx& c) {
   if (c.device >= 0) (void)hipSetDevice(c.device);
+  fhip::fused_release(c);
   for (auto& row : c.graph)
     for (auto& g : row)
       if (g) (void)hipGraphExecDestroy(g);
@@ -236,8 +237,7 @@ int fhip_create(const fhip_desc* d, fhip_ctx** out) {
       c.flag = dcopy<int>(nullptr, 1, c.stream);
       c.h_dt = p.dt;
       c.iter = p.init_iter;
-      c.fused_ok = fhip::fused_supported(c);
-      if (c.fused_ok) fhip::fused_setup(c);
+      c.fused_ok = fhip::fused_setup(c, d);
       HCHK(hipStreamSynchronize(c.stream));
     } catch (...) {
       destroy_ctx(c);
@@ -392,6 +392,7 @@ int fhip_update(fhip_ctx* h, int n_steps, int path) {
     bool fused = (path == FHIP_PATH_FUSED) || (path == FHIP_PATH_AUTO && c.fused_ok);
     require(!fused || c.fused_ok, "the fused path does not support this configuration");
     const int nRK = c.z.nRK;
+    if (fused && n_steps > 0) fhip::fused_begin(c);
     for (int it = 0; it < n_steps; it++) {
       c.iter++;
       bool refresh = (c.iter == c.prm.init_iter + 1);
@@ -415,6 +416,7 @@ int fhip_update(fhip_ctx* h, int n_steps, int path) {
         fhip::staged_time_step_a(c, nRK - 1, c.prm.rk_a[nRK], false);
       }
     }
+    if (fused && n_steps > 0) fhip::fused_end(c);
     return 0;
   });
 }

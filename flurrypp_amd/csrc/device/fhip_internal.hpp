@@ -63,6 +63,7 @@ struct Ctx {
   // captured RK steps (fused path): index 0 without, 1 with wave-speed refresh
   hipGraphExec_t graph[2][3] = {{nullptr, nullptr, nullptr}, {nullptr, nullptr, nullptr}};
   bool fused_ok = false;
+  void* fused = nullptr;  // FusedState (fhip_fused.hip)
 };
 
 // Launch helper: records per-kernel time with events when timing is on.
@@ -97,8 +98,10 @@ void residual_norm(Ctx& c, int res_type, double* out_host);
 int check_state(Ctx& c);
 
 // fused path (fhip_fused.hip)
-bool fused_supported(const Ctx& c);
-void fused_setup(Ctx& c);
+bool fused_setup(Ctx& c, const fhip_desc* d);  // false: configuration not covered
+void fused_release(Ctx& c);
+void fused_begin(Ctx& c);  // import U and wave speeds, extrapolate U to the faces
+void fused_end(Ctx& c);    // export U, divF_spts[0] and wave speeds
 void fused_rk_step(Ctx& c, bool refresh);
 
 }  // namespace fhip
