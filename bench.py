@@ -1,0 +1,170 @@
+#!/usr/bin/env python3
+"""Benchmark of the FR residual + RK update on one MI355X.
+
+Workload (default): 3-D compressible Navier-Stokes Taylor-Green vortex,
+48^3 periodic hex mesh on [0,2pi]^3, P=3 (64 solution points per element,
+7,077,888 DOF x 5 fields), Rusanov inviscid + LDG viscous flux, Re=1600,
+Mach 0.1, Pr=0.71, RK44, fixed dt from a CFL-0.5 evaluation at t=0, FP64.
+One benchmark step is one RK44 time step (4 stages).  The metric counts
+DOF x stages per second (DOF = elements x solution points).
+
+Prints one JSON line.  The timed region runs `--steps` RK steps through
+fhip_update on the device, with every kernel bracketed by HIP events on the
+context's stream (no synchronisation inside the region); the per-kernel
+averages give the roofline fraction of the dominant kernel.  The CPU
+baseline times the C restatement in oracle/ (serial) on a smaller Taylor-Green
+mesh on this host.
+"""
+from __future__ import annotations
+
+import argparse
+import json
+import os
+import sys
+import time
+from pathlib import Path
+
+import numpy as np
+
+ROOT = Path(__file__).resolve().parent
+sys.path.insert(0, str(ROOT))
+
+METRIC = "GDOF·stage/s at P3 hex TGV, FP64 vs mixed; p50/p99 solve latency batched"
+HBM_PEAK_GBS = 8000.0
+FP64_PEAK_TFS = 78.6
+
+
+def log(msg: str) -> None:
+    print(msg, file=sys.stderr, flush=True)
+
+
+def cpu_baseline(n: int, steps: int) -> dict:
+    from flurrypp_amd import cases
+    from oracle.oracle import Oracle
+    setup, U = cases.tgv3d(n=n, order=3, viscous=1)
+    o = Oracle.from_setup(setup)
+    o.initializeSolution(U, 0)
+    t0 = time.perf_counter()
+    o.update(steps)
+    el = time.perf_counter() - t0
+    dof = setup.nEles * setup.nSpts
+    return {"value": dof * 4 * steps / el / 1e9, "unit": "GDOF·stage/s", "cores": 1, "kind": "port",
+            "sample": f"C restatement (oracle/, serial, gcc -O3) on the same Taylor-Green NS case at {n}^3 hex P3, "
+                      f"{steps} RK44 steps ({dof} DOF), {el:.1f} s"}
+
+
+def traffic_for(kernel: str, workload: str):
+    """HBM bytes per launch from the committed rocprofv3 --pmc summary."""
+    p = ROOT / "profiles" / "traffic.json"
+    if not p.exists():
+        return None
+    try:
+        d = json.loads(p.read_text())
+        ent = d.get(workload, {}).get(kernel)
+        return None if ent is None else float(ent["bytes_per_launch"])
+    except Exception:
+        return None
+
+
+def main() -> int:
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=50)
+    ap.add_argument("--warmup", type=int, default=3)
+    ap.add_argument("--n", type=int, default=48, help="elements per direction")
+    ap.add_argument("--order", type=int, default=3)
+    ap.add_argument("--inviscid", action="store_true")
+    ap.add_argument("--path", choices=["fused", "staged"], default="fused")
+    ap.add_argument("--cpu-n", type=int, default=12)
+    ap.add_argument("--cpu-steps", type=int, default=8)
+    ap.add_argument("--no-cpu", action="store_true")
+    a = ap.parse_args()
+
+    if a.gpus != 1:
+        print(f"bench.py runs on one GPU only; --gpus {a.gpus} is not supported "
+              "(multi-GPU sharding of this path is described in DESIGN.md).", file=sys.stderr)
+        return 2
+
+    from flurrypp_amd import cases, perfmodel
+    from flurrypp_amd._native import PATH_FUSED, PATH_STAGED
+    from flurrypp_amd.solver import Solver
+
+    visc = 0 if a.inviscid else 1
+    log(f"bench: setting up TGV {a.n}^3 P{a.order} {'NS' if visc else 'Euler'}")
+    t = time.perf_counter()
+    setup, U = cases.tgv3d(n=a.n, order=a.order, viscous=visc)
+    log(f"bench: host set-up {time.perf_counter() - t:.1f} s, dt={setup.params.dt:.6e}")
+    s = Solver(setup.params, setup=setup)
+    s.initializeSolution(U)
+    path = PATH_FUSED if a.path == "fused" else PATH_STAGED
+    s.update(a.warmup, path=path)
+    s.check_state()
+    log("bench: warmup done")
+
+    s.enable_timing(True)
+    s.sync()
+    t0 = time.perf_counter()
+    s.update(a.steps, path=path, sync=False)
+    s.sync()
+    el = time.perf_counter() - t0
+    kt = s.kernel_times()
+    s.enable_timing(False)
+    s.check_state()
+
+    nRK = setup.params.nRKSteps
+    dof = setup.nEles * setup.nSpts
+    value = dof * nRK * a.steps / el / 1e9
+    workload = f"tgv3d_{a.n}_p{a.order}_{'ns' if visc else 'euler'}_{a.path}"
+
+    # roofline of the dominant kernel
+    nFF = len(setup.faces.eleL) * setup.nFptsFace
+    nFFb = setup.faces.nBnd * setup.nFptsFace
+    work = perfmodel.fused_work(setup.nDims, setup.order, setup.nEles, nFF, nFFb, bool(visc), nRK,
+                                setup.params.timeType)
+    kernels = {}
+    for name, (avg_ms, cnt) in kt.items():
+        ent = {"avg_ms": round(avg_ms, 5), "launches": cnt}
+        w = work.get(name)
+        if w is not None and avg_ms > 0:
+            ent["GB/s"] = round(w.bytes / (avg_ms * 1e-3) / 1e9, 1)
+            ent["bytes_per_launch"] = w.bytes
+        kernels[name] = ent
+    dom = max((k for k in kernels if k in work), key=lambda k: kernels[k]["avg_ms"] * kernels[k]["launches"],
+              default=None)
+    roof = None
+    if dom:
+        w = work[dom]
+        avg_s = kernels[dom]["avg_ms"] * 1e-3
+        if perfmodel.hbm_bound(w):
+            ach = w.bytes / avg_s / 1e9
+            roof = {"bound": "hbm", "achieved": round(ach, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                    "frac": round(ach / HBM_PEAK_GBS, 4), "traffic": traffic_for(dom, workload), "kernel": dom}
+        else:
+            ach = w.flops / avg_s / 1e12
+            roof = {"bound": "mfma", "achieved": round(ach, 2), "peak": FP64_PEAK_TFS, "unit": "TFLOP/s",
+                    "frac": round(ach / FP64_PEAK_TFS, 4), "traffic": traffic_for(dom, workload), "kernel": dom}
+
+    cpu = None
+    if not a.no_cpu:
+        log("bench: CPU baseline (oracle restatement)")
+        cpu = cpu_baseline(a.cpu_n, a.cpu_steps)
+
+    out = {
+        "metric": METRIC, "value": round(value, 4), "unit": "GDOF·stage/s", "n_gpus": 1, "steps": a.steps,
+        "warmup": a.warmup, "ms_per_step": round(el / a.steps * 1e3, 4), "higher_is_better": True,
+        "scaling": "weak", "vs_baseline": None, "dtype": "f64", "data": "synthetic",
+        "config": {"workload": f"3D {'Navier-Stokes' if visc else 'Euler'} Taylor-Green vortex, {a.n}^3 periodic "
+                               f"hex, P{a.order}, Rusanov{'+LDG' if visc else ''}, RK44, FP64",
+                   "elements": setup.nEles, "dof": dof, "stages_per_step": nRK, "path": a.path,
+                   "dt": setup.params.dt},
+        "roofline": roof,
+        "cpu_baseline": cpu,
+        "kernels": kernels,
+    }
+    print(json.dumps(out), flush=True)
+    s.close()
+    return 0
+
+
+if __name__ == "__main__":
+    sys.exit(main())

@@ -83,6 +83,7 @@ void destroy_ctx(fhip::Ctx& c) {
                   c.divF, c.waveSp, c.eleDt, c.faceFn, c.src, c.Uacc, c.d_dt, c.partial, c.flag};
   for (void* p : ptrs)
     if (p) (void)hipFree(p);
+  for (hipEvent_t e : c.timer.pool) (void)hipEventDestroy(e);
   if (c.ev0) (void)hipEventDestroy(c.ev0);
   if (c.ev1) (void)hipEventDestroy(c.ev1);
   if (c.stream) (void)hipStreamDestroy(c.stream);
@@ -111,6 +112,26 @@ double* array_ptr(fhip::Ctx& c, int which, int step, long long* n) {
 }
 
 }  // namespace
+
+namespace fhip {
+void timer_resolve(Ctx& c) {
+  Timer& t = c.timer;
+  if (t.pending.empty()) {
+    t.used = 0;
+    return;
+  }
+  (void)hipStreamSynchronize(c.stream);
+  for (auto& pr : t.pending) {
+    float ms = 0;
+    (void)hipEventElapsedTime(&ms, t.pool[pr.second], t.pool[pr.second + 1]);
+    auto& a = t.acc[pr.first];
+    a.first += ms;
+    a.second += 1;
+  }
+  t.pending.clear();
+  t.used = 0;
+}
+}  // namespace fhip
 
 extern "C" {
 
@@ -530,6 +551,7 @@ int fhip_stream(fhip_ctx* h, void** stream) {
 
 int fhip_enable_timing(fhip_ctx* h, int on) {
   if (!h) return FHIP_EINVAL;
+  fhip::timer_resolve(h->c);
   h->c.timing = on != 0;
   h->c.timer.acc.clear();
   return 0;
@@ -537,6 +559,7 @@ int fhip_enable_timing(fhip_ctx* h, int on) {
 
 int fhip_kernel_times(fhip_ctx* h, int max, char* names, int name_len, double* avg_ms, long long* counts) {
   if (!h) return -1;
+  fhip::timer_resolve(h->c);
   int i = 0;
   for (auto& kv : h->c.timer.acc) {
     if (i >= max) break;

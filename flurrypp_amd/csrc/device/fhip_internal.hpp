@@ -27,8 +27,14 @@ struct FaceTables {
   int* bc = nullptr;    // [nFaceFpts], -1 interior
 };
 
+// Per-kernel timing with HIP events recorded on the context's stream around
+// each launch; nothing synchronises until the times are read (or the event
+// pool is full), so timing runs inside the timed region itself.
 struct Timer {
   std::map<std::string, std::pair<double, long long>> acc;  // total ms, launches
+  std::vector<hipEvent_t> pool;
+  std::vector<std::pair<const char*, size_t>> pending;       // name, index of start event
+  size_t used = 0;
 };
 
 struct Ctx {
@@ -66,23 +72,30 @@ struct Ctx {
   void* fused = nullptr;  // FusedState (fhip_fused.hip)
 };
 
-// Launch helper: records per-kernel time with events when timing is on.
+void timer_resolve(Ctx& c);
+
+// Launch helper: brackets one kernel launch with timing events when enabled.
 struct Launch {
   Ctx& c;
   const char* name;
+  size_t slot = 0;
   Launch(Ctx& c_, const char* n) : c(c_), name(n) {
-    if (c.timing) (void)hipEventRecord(c.ev0, c.stream);
+    if (!c.timing) return;
+    Timer& t = c.timer;
+    if (t.used + 2 > 8192) timer_resolve(c);
+    while (t.pool.size() < t.used + 2) {
+      hipEvent_t e;
+      (void)hipEventCreate(&e);
+      t.pool.push_back(e);
+    }
+    slot = t.used;
+    t.used += 2;
+    (void)hipEventRecord(t.pool[slot], c.stream);
   }
   ~Launch() {
-    if (c.timing) {
-      (void)hipEventRecord(c.ev1, c.stream);
-      (void)hipEventSynchronize(c.ev1);
-      float ms = 0;
-      (void)hipEventElapsedTime(&ms, c.ev0, c.ev1);
-      auto& a = c.timer.acc[name];
-      a.first += ms;
-      a.second += 1;
-    }
+    if (!c.timing) return;
+    (void)hipEventRecord(c.timer.pool[slot + 1], c.stream);
+    c.timer.pending.push_back({name, slot});
   }
 };
 

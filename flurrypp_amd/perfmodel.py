@@ -1,0 +1,82 @@
+"""Algorithmic work of each libflurry_hip kernel, for roofline fractions.
+
+Bytes count every array a kernel must read or write once at its storage
+precision (FP64 state, metrics and fluxes; int32 face indices); flops count
+the arithmetic of the operator contractions only (flux formulas are a few
+hundred flops per point and never bound these kernels).  Counts are per
+launch of the kernel on a mesh with E elements and nFF face flux points of
+which nFFb lie on boundaries.
+"""
+from __future__ import annotations
+
+from dataclasses import dataclass
+
+B8, B4 = 8, 4  # bytes per double / int32
+
+
+@dataclass
+class Work:
+    bytes: float
+    flops: float
+
+
+def fused_work(D: int, P: int, E: int, nFF: int, nFFb: int, viscous: bool, nRK: int, time_type: int) -> dict:
+    """Per-launch algorithmic bytes/flops of the fused-path kernels."""
+    N, P1 = D + 2, P + 1
+    S = P1 ** D
+    F = 2 * D * P1 ** (D - 1)
+    nFFi = nFF - nFFb
+    out = {}
+    # face_inviscid: read UL, UR (interior), normal; write inviscid Fn (or Fn*dA
+    # on both sides), wave speed; 3 int32 per face flux point
+    rd = nFF * (N + D + 3 * B4 / B8) + nFFi * N
+    if viscous:
+        wr = nFF * (N + N + 1) + nFFi * N          # FnInv, dUc L (+R), waveSp
+    else:
+        rd += nFF + nFFi                             # dA L (+R)
+        wr = nFF * (N + 1) + nFFi * N                # Fn L (+R), waveSp
+    out["fused_face_inviscid"] = Work((rd + wr) * B8, 0.0)
+    # volume
+    rd = E * (N * S + D * D * S)
+    wr = E * (N * S + N * F)
+    fl = E * N * (D * S * 2 * P1 + F * D * 2 * P1)   # divergence + normal-flux extrapolation
+    if viscous:
+        rd += E * (S + N * F)                        # detJ, dUc
+        wr += E * D * N * F                          # gradient at the faces
+        fl += E * N * (D * S * 2 * P1 + D * S * 2 * 2 * D + D * F * 2 * P1)
+        out["fused_volume_viscous"] = Work((rd + wr) * B8, fl)
+    else:
+        out["fused_volume_inviscid"] = Work((rd + wr) * B8, fl)
+    if viscous:
+        rd = nFF * (N + D * N + N + D + 1 + 3 * B4 / B8) + nFFi * (N + D * N + 1)
+        wr = nFF * N + nFFi * N
+        out["fused_face_viscous"] = Work((rd + wr) * B8, 0.0)
+    # rk update, averaged over the stages of one step
+    per_stage = []
+    for st in range(nRK):
+        rd = E * (N * S + 2 * N * F + S)             # divF, Fn, disFn, detJ
+        wr = E * (N * S + N * F)                     # U, U at the faces
+        if time_type >= 5:
+            rd += E * N * S                          # U0
+        elif nRK == 1:
+            rd += E * N * S                          # U
+        elif st == 0:
+            rd += E * N * S                          # U
+            wr += 2 * E * N * S                      # U0, Uacc
+        elif st == nRK - 1:
+            rd += E * N * S                          # Uacc
+        else:
+            rd += 2 * E * N * S                      # U0, Uacc
+            wr += E * N * S                          # Uacc
+        if st == 0:
+            wr += E * N * S                          # divF_spts[0]
+        per_stage.append((rd + wr) * B8)
+    out["fused_rk_update"] = Work(sum(per_stage) / len(per_stage), E * N * S * 2 * 2 * D)
+    return out
+
+
+def hbm_bound(w: Work, peak_tf: float = 78.6, peak_bw: float = 8.0) -> bool:
+    """True when the kernel's intensity is under the FP64 ridge point."""
+    if w.bytes == 0:
+        return False
+    return (w.flops / w.bytes) < (peak_tf / peak_bw)

@@ -881,3 +881,30 @@ int or_residual_norm(or_ctx* c, int res_type, double* out) {
     for (int k = 0; k < N; k++) out[k] = sqrt(out[k]);
   return 0;
 }
+
+/* ---- pointwise hooks for the oracle's own tests ------------------------- */
+
+void or_hook_inviscid_flux(const or_params* p, const double* U, double* F /* [3][5] */) {
+  double f[3][5] = {{0}};
+  inviscid_flux(U, f, p);
+  memcpy(F, f, sizeof f);
+}
+
+void or_hook_viscous_flux(const or_params* p, const double* U, const double* g /* [3][5] */, double* Fv) {
+  double gg[3][5], f[3][5] = {{0}};
+  memcpy(gg, g, sizeof gg);
+  viscous_flux(U, gg, f, p);
+  memcpy(Fv, f, sizeof f);
+}
+
+double or_hook_rusanov(const or_params* p, const double* UL, const double* UR, const double* n, double* Fn) {
+  return rusanov(p, UL, UR, n, Fn);
+}
+
+void or_hook_roe2d(const or_params* p, const double* UL, const double* UR, const double* n, double* Fn) {
+  roe2d(p, UL, UR, n, Fn);
+}
+
+void or_hook_bc(const or_params* p, int bc, const double* UL, const double* n, double* UR) { apply_bcs(p, bc, UL, n, UR); }
+
+double or_hook_ldg_pen(const or_params* p, const double* n) { return ldg_pen(p, n); }

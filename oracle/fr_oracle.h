@@ -68,6 +68,16 @@ int or_update(or_ctx* c, int n_steps, const double* src);
 int or_residual_norm(or_ctx* c, int res_type, double* out);
 int or_get_iter(or_ctx* c);
 
+/* Pointwise hooks used by the oracle's own tests (src/flux.cpp:37-210,
+ * src/face.cpp:314-495,588-615, src/boundFace.cpp:65-314).  F arrays are
+ * [3][5] row-major. */
+void or_hook_inviscid_flux(const or_params* p, const double* U, double* F);
+void or_hook_viscous_flux(const or_params* p, const double* U, const double* g, double* Fv);
+double or_hook_rusanov(const or_params* p, const double* UL, const double* UR, const double* n, double* Fn);
+void or_hook_roe2d(const or_params* p, const double* UL, const double* UR, const double* n, double* Fn);
+void or_hook_bc(const or_params* p, int bc, const double* UL, const double* n, double* UR);
+double or_hook_ldg_pen(const or_params* p, const double* n);
+
 #ifdef __cplusplus
 }
 #endif
