@@ -53,6 +53,15 @@ def cpu_baseline(n: int, steps: int) -> dict:
                       f"{steps} RK44 steps ({dof} DOF), {el:.1f} s"}
 
 
+def kinetic_energy(setup, U) -> float:
+    """E_k = (1/|Omega|) sum w detJ rho |u|^2 / 2 over solution points (FP64)."""
+    D = setup.nDims
+    wj = np.einsum("s,se->se", setup.opers.w_spts, setup.geo.detjac_spts)
+    rho = U[..., 0]
+    ke = 0.5 * np.sum(U[..., 1:D + 1] ** 2, axis=-1) / rho
+    return float(np.sum(wj * ke) / np.sum(wj))
+
+
 def traffic_for(kernel: str, workload: str):
     """HBM bytes per launch from the committed rocprofv3 --pmc summary."""
     p = ROOT / "profiles" / "traffic.json"
@@ -78,6 +87,7 @@ def main() -> int:
     ap.add_argument("--cpu-n", type=int, default=12)
     ap.add_argument("--cpu-steps", type=int, default=8)
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--no-mixed", action="store_true")
     a = ap.parse_args()
 
     if a.gpus != 1:
@@ -144,6 +154,37 @@ def main() -> int:
             roof = {"bound": "mfma", "achieved": round(ach, 2), "peak": FP64_PEAK_TFS, "unit": "TFLOP/s",
                     "frac": round(ach / FP64_PEAK_TFS, 4), "traffic": traffic_for(dom, workload), "kernel": dom}
 
+    forms = {}
+    if a.path == "fused" and not a.no_mixed:
+        # mixed-precision form on the same case, from the same initial state
+        from flurrypp_amd._native import PATH_MIXED
+        U64 = s.get_U()
+        s.set_U(U)
+        s.update(a.warmup, path=PATH_MIXED)
+        s.enable_timing(True)
+        s.sync()
+        t0 = time.perf_counter()
+        s.update(a.steps, path=PATH_MIXED, sync=False)
+        s.sync()
+        elm = time.perf_counter() - t0
+        ktm = s.kernel_times()
+        s.enable_timing(False)
+        U32 = s.get_U()
+        rho_err = float(np.linalg.norm(U32[..., 0] - U64[..., 0]) / np.linalg.norm(U64[..., 0]))
+        u_err = float(np.linalg.norm(U32 - U64) / np.linalg.norm(U64))
+        T = (a.warmup + a.steps) * setup.params.dt
+        ek0, ek64, ek32 = (kinetic_energy(setup, x) for x in (U, U64, U32))
+        eps64, eps32 = -(ek64 - ek0) / T, -(ek32 - ek0) / T
+        forms["mixed"] = {
+            "value": round(dof * nRK * a.steps / elm / 1e9, 4), "unit": "GDOF·stage/s",
+            "ms_per_step": round(elm / a.steps * 1e3, 4), "speedup_vs_fp64": round(el / elm, 3),
+            "rel_l2_density_vs_fp64": rho_err, "rel_l2_state_vs_fp64": u_err,
+            "ke_dissipation_rate_fp64": eps64, "ke_dissipation_rate_mixed": eps32,
+            "ke_dissipation_rel_err": abs(eps32 - eps64) / abs(eps64) if eps64 else None,
+            "after_steps": a.warmup + a.steps,
+            "kernels": {k: {"avg_ms": round(v[0], 5), "launches": v[1]} for k, v in ktm.items()},
+        }
+
     cpu = None
     if not a.no_cpu:
         log("bench: CPU baseline (oracle restatement)")
@@ -160,6 +201,7 @@ def main() -> int:
         "roofline": roof,
         "cpu_baseline": cpu,
         "kernels": kernels,
+        "forms": forms,
     }
     print(json.dumps(out), flush=True)
     s.close()

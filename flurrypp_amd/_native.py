@@ -54,7 +54,7 @@ ARR = {
     "dU_spts": 6, "dU_fpts": 7, "dUc_fpts": 8, "U0": 9, "waveSp": 10, "ele_dt": 11,
     "face_fpt_L": 12, "face_fpt_R": 13,
 }
-PATH_AUTO, PATH_STAGED, PATH_FUSED = 0, 1, 2
+PATH_AUTO, PATH_STAGED, PATH_FUSED, PATH_MIXED = 0, 1, 2, 3
 
 
 def dptr(a: np.ndarray | None):
@@ -102,6 +102,7 @@ def hip_lib() -> C.CDLL:
         lib.fhip_get_iter.argtypes = [C.c_void_p, _ip]
         lib.fhip_update.argtypes = [C.c_void_p, C.c_int, C.c_int]
         lib.fhip_path_supported.argtypes = [C.c_void_p, C.c_int]
+        lib.fhip_set_option.argtypes = [C.c_void_p, C.c_int, C.c_int]
         lib.fhip_residual_norm.argtypes = [C.c_void_p, C.c_int, _dp]
         lib.fhip_array_size.argtypes = [C.c_void_p, C.c_int]
         lib.fhip_array_size.restype = C.c_longlong

@@ -408,17 +408,34 @@ int fhip_update(fhip_ctx* h, int n_steps, int path) {
   return guard([&] {
     require(h, "null context");
     require(n_steps >= 0, "n_steps must be >= 0");
-    require(path == FHIP_PATH_AUTO || path == FHIP_PATH_STAGED || path == FHIP_PATH_FUSED, "unknown path");
+    require(path == FHIP_PATH_AUTO || path == FHIP_PATH_STAGED || path == FHIP_PATH_FUSED || path == FHIP_PATH_MIXED,
+            "unknown path");
     fhip::Ctx& c = h->c;
-    bool fused = (path == FHIP_PATH_FUSED) || (path == FHIP_PATH_AUTO && c.fused_ok);
-    require(!fused || c.fused_ok, "the fused path does not support this configuration");
+    const bool mixed = path == FHIP_PATH_MIXED;
+    bool fused = mixed || (path == FHIP_PATH_FUSED) || (path == FHIP_PATH_AUTO && c.fused_ok);
+    require(!fused || c.fused_ok, "the fused paths do not support this configuration");
     const int nRK = c.z.nRK;
-    if (fused && n_steps > 0) fhip::fused_begin(c);
+    if (fused && n_steps > 0) fhip::fused_begin(c, mixed);
     for (int it = 0; it < n_steps; it++) {
       c.iter++;
       bool refresh = (c.iter == c.prm.init_iter + 1);
       if (fused) {
-        fhip::fused_rk_step(c, refresh);
+        // Steady-state steps replay one captured graph; the wave-speed
+        // refresh step and timed runs launch kernel by kernel.
+        if (!refresh && !c.timing && c.use_graphs) {
+          hipGraphExec_t& ge = c.graph[0][mixed ? 1 : 0];
+          if (!ge) {
+            hipGraph_t g = nullptr;
+            HCHK(hipStreamBeginCapture(c.stream, hipStreamCaptureModeThreadLocal));
+            fhip::fused_rk_step(c, false, mixed);
+            HCHK(hipStreamEndCapture(c.stream, &g));
+            HCHK(hipGraphInstantiate(&ge, g, nullptr, nullptr, 0));
+            (void)hipGraphDestroy(g);
+          }
+          HCHK(hipGraphLaunch(ge, c.stream));
+        } else {
+          fhip::fused_rk_step(c, refresh, mixed);
+        }
         continue;
       }
       // solver::update (src/solver.cpp:319-363), static grid
@@ -445,8 +462,17 @@ int fhip_update(fhip_ctx* h, int n_steps, int path) {
 int fhip_path_supported(fhip_ctx* h, int path) {
   if (!h) return 0;
   if (path == FHIP_PATH_STAGED || path == FHIP_PATH_AUTO) return 1;
-  if (path == FHIP_PATH_FUSED) return h->c.fused_ok ? 1 : 0;
+  if (path == FHIP_PATH_FUSED || path == FHIP_PATH_MIXED) return h->c.fused_ok ? 1 : 0;
   return 0;
+}
+
+int fhip_set_option(fhip_ctx* h, int option, int value) {
+  return guard([&] {
+    require(h, "null context");
+    if (option == FHIP_OPT_GRAPHS) h->c.use_graphs = value != 0;
+    else throw ApiError(FHIP_EINVAL, "unknown option");
+    return 0;
+  });
 }
 
 int fhip_sync(fhip_ctx* h) {

@@ -1,17 +1,16 @@
-// Fused tensor-product ("fused") path.
+// Fused tensor-product paths: FP64 ("fused") and mixed precision ("mixed").
 //
 // FlurryPP applies its FR operators as dense GEMMs (src/solver.cpp:556-1158),
 // but on quads and hexes every operator row is non-zero only along one line
 // of solution points: the Lagrange factors of the other directions are
-// evaluated at their own nodes and are exactly 0 or 1.  This path reads the
-// dense operators handed to fhip_create, keeps only their non-zero entries
+// evaluated at their own nodes and are exactly 0 or 1.  These paths read the
+// dense operators handed to fhip_create, keep only their non-zero entries
 // (at most P+1 per row for extrapolation and gradient, one per face for the
-// correction) and applies them inside fused kernels, so the sums are the same
-// as the dense products minus terms that are exactly zero.
+// correction) and apply them inside fused kernels, so the sums are the dense
+// products minus terms that are exactly zero.
 //
 // Data live element-major on the device ([ele][field][point]) so that a block
-// of elements is one contiguous slab: an element's solution (or face) points
-// are read and written with unit stride.  Per RK stage:
+// of elements is one contiguous slab.  Per RK stage:
 //   face_inv   common inviscid flux at every face flux point (+ LDG dUc)
 //   volume     [gradient + correction + physical transform], inviscid (and
 //              viscous) flux, transformed flux, divergence, discontinuous
@@ -19,6 +18,11 @@
 //   face_visc  LDG common viscous flux (viscous runs only)
 //   rk         correction of the divergence, RK update, U at the faces of
 //              the next stage
+//
+// Every kernel is templated on R, the type of the face and volume
+// intermediates and of the flux arithmetic: double for the FP64 path, float
+// for the mixed path.  The solution, its RK stage copies, the metric terms,
+// the wave speeds, dt and the residual array stay FP64 in both.
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
@@ -47,7 +51,7 @@ struct Shape {
   static constexpr int S = (D == 2) ? P1 * P1 : P1 * P1 * P1;
   static constexpr int F = 2 * D * Ff;
   static constexpr int N = D + 2;
-  static constexpr int NC = 2 * D;              // correction entries per solution point
+  static constexpr int NC = 2 * D;                          // correction entries per solution point
   static constexpr int EB = (256 / S) > 0 ? (256 / S) : 1;  // elements per block
   static constexpr int NT = 256;
 };
@@ -64,22 +68,43 @@ struct Tabs {
   const double* cu_w;   // [D][S][NC]  opp_correct_u entries
 };
 
+// Face and volume intermediates of one precision.
+template <class R>
+struct Work {
+  R *divFe = nullptr, *Ufe = nullptr, *Fne = nullptr, *disFne = nullptr, *dUce = nullptr, *dUfe = nullptr,
+    *FnInv = nullptr;
+  bool ready = false;
+  void release() {
+    void* ps[] = {divFe, Ufe, Fne, disFne, dUce, dUfe, FnInv};
+    for (void* p : ps)
+      if (p) (void)hipFree(p);
+    *this = Work<R>();
+  }
+};
+
 struct FusedState {
-  int P1 = 0;
   Tabs tb{};
   void* tab_mem = nullptr;
-  // element-major arrays
-  double *Ue = nullptr, *U0e = nullptr, *Uacc = nullptr, *divFe = nullptr, *divF0e = nullptr, *Ufe = nullptr,
-         *Fne = nullptr, *disFne = nullptr, *dUce = nullptr, *dUfe = nullptr, *FnInv = nullptr, *JGe = nullptr,
-         *detJe = nullptr, *norme = nullptr, *dAe = nullptr, *waveSpe = nullptr;
+  // FP64 element-major state, metrics and wave speeds (both paths)
+  double *Ue = nullptr, *U0e = nullptr, *Uacc = nullptr, *divF0e = nullptr, *JGe = nullptr, *detJe = nullptr,
+         *norme = nullptr, *dAe = nullptr, *waveSpe = nullptr;
   int *pL = nullptr, *pR = nullptr, *bc = nullptr;
+  Work<double> w64;
+  Work<float> w32;
   ~FusedState() {
-    void* ps[] = {tab_mem, Ue, U0e, Uacc, divFe, divF0e, Ufe, Fne, disFne, dUce, dUfe, FnInv, JGe, detJe, norme, dAe,
-                  waveSpe, pL, pR, bc};
+    w64.release();
+    w32.release();
+    void* ps[] = {tab_mem, Ue, U0e, Uacc, divF0e, JGe, detJe, norme, dAe, waveSpe, pL, pR, bc};
     for (void* p : ps)
       if (p) (void)hipFree(p);
   }
+  template <class R>
+  Work<R>& work();
 };
+template <>
+Work<double>& FusedState::work<double>() { return w64; }
+template <>
+Work<float>& FusedState::work<float>() { return w32; }
 
 // ---------------------------------------------------------------------------
 // Layout conversions between FlurryPP's [pt][ele][k] and element-major.
@@ -106,22 +131,22 @@ __global__ void k_from_em(long long total, int P, int E, int K, const double* __
 
 // ---------------------------------------------------------------------------
 // U at the flux points from U at the solution points (opp_spts_to_fpts rows).
-template <int D, int P>
-__global__ __launch_bounds__(256) void k_extrap(int E, const double* __restrict__ Ue, double* __restrict__ Ufe, Tabs tb) {
+template <int D, int P, class R>
+__global__ __launch_bounds__(256) void k_extrap(int E, const double* __restrict__ Ue, R* __restrict__ Ufe, Tabs tb) {
   using Sh = Shape<D, P>;
   constexpr int S = Sh::S, F = Sh::F, N = Sh::N, EB = Sh::EB, P1 = Sh::P1;
-  __shared__ double sU[EB * N * S];
+  __shared__ R sU[EB * N * S];
   const int e0 = blockIdx.x * EB, nb = min(EB, E - e0);
   const double* src = Ue + (long long)e0 * N * S;
-  for (int i = threadIdx.x; i < nb * N * S; i += blockDim.x) sU[i] = src[i];
+  for (int i = threadIdx.x; i < nb * N * S; i += blockDim.x) sU[i] = R(src[i]);
   __syncthreads();
   for (int i = threadIdx.x; i < nb * F; i += blockDim.x) {
     int b = i / F, f = i - b * F;
 #pragma unroll
     for (int k = 0; k < N; k++) {
-      double acc = 0.;
+      R acc = 0;
 #pragma unroll
-      for (int m = 0; m < P1; m++) acc += tb.ext_w[f * P1 + m] * sU[(b * N + k) * S + tb.ext_idx[f * P1 + m]];
+      for (int m = 0; m < P1; m++) acc += R(tb.ext_w[f * P1 + m]) * sU[(b * N + k) * S + tb.ext_idx[f * P1 + m]];
       Ufe[((long long)(e0 + b) * N + k) * F + f] = acc;
     }
   }
@@ -129,12 +154,12 @@ __global__ __launch_bounds__(256) void k_extrap(int E, const double* __restrict_
 
 // ---------------------------------------------------------------------------
 // Common inviscid flux at face flux points; pL/pR are ele*F+fpt.
-template <int D, int P>
+template <int D, int P, class R>
 __global__ __launch_bounds__(256) void k_face_inv(int nFF, const int* __restrict__ pLa, const int* __restrict__ pRa,
-                                                   const int* __restrict__ bca, const double* __restrict__ Ufe,
+                                                   const int* __restrict__ bca, const R* __restrict__ Ufe,
                                                    const double* __restrict__ norme, const double* __restrict__ dAe,
-                                                   double* __restrict__ Fne, double* __restrict__ FnInv,
-                                                   double* __restrict__ waveSpe, double* __restrict__ dUce, Phys ph) {
+                                                   R* __restrict__ Fne, R* __restrict__ FnInv,
+                                                   double* __restrict__ waveSpe, R* __restrict__ dUce, Phys ph) {
   using Sh = Shape<D, P>;
   constexpr int F = Sh::F, N = Sh::N;
   int t = blockIdx.x * blockDim.x + threadIdx.x;
@@ -142,40 +167,40 @@ __global__ __launch_bounds__(256) void k_face_inv(int nFF, const int* __restrict
   const int pl = pLa[t], pr = pRa[t], bc = bca[t];
   const long long eL = pl / F, fL = pl - eL * F;
   const long long bL = eL * N * F + fL;
-  double UL[N], UR[N], n[D], fn[N];
+  R UL[N], UR[N], n[D], fn[N];
 #pragma unroll
   for (int k = 0; k < N; k++) UL[k] = Ufe[bL + k * F];
 #pragma unroll
-  for (int d = 0; d < D; d++) n[d] = norme[(eL * D + d) * F + fL];
+  for (int d = 0; d < D; d++) n[d] = R(norme[(eL * D + d) * F + fL]);
   long long bR = 0;
   if (pr < 0) {
-    apply_bc<D, double>(bc, UL, n, UR, ph);
+    apply_bc<D, R>(bc, UL, n, UR, ph);
   } else {
     const long long eR = pr / F, fR = pr - eR * F;
     bR = eR * N * F + fR;
 #pragma unroll
     for (int k = 0; k < N; k++) UR[k] = Ufe[bR + k * F];
   }
-  if (pr < 0 || ph.riemann_type == 0) waveSpe[pl] = rusanov<D, double>(UL, UR, n, fn, ph.gamma);
+  if (pr < 0 || ph.riemann_type == 0) waveSpe[pl] = double(rusanov<D, R>(UL, UR, n, fn, R(ph.gamma)));
   else {
-    if constexpr (D == 2) roe2d<double>(UL, UR, n, fn, ph.gamma);
+    if constexpr (D == 2) roe2d<R>(UL, UR, n, fn, R(ph.gamma));
   }
   if (ph.viscous) {
 #pragma unroll
     for (int k = 0; k < N; k++) FnInv[(long long)k * nFF + t] = fn[k];
-    double pf = (pr < 0) ? 0.0 : ldg_pen<D, double>(n, ph.pen_fact);
+    R pf = (pr < 0) ? R(0) : ldg_pen<D, R>(n, R(ph.pen_fact));
 #pragma unroll
     for (int k = 0; k < N; k++) {
-      double uc = (pr < 0) ? 0.5 * (UL[k] + UR[k]) : 0.5 * (UL[k] + UR[k]) - pf * (UL[k] - UR[k]);
+      R uc = (pr < 0) ? R(0.5) * (UL[k] + UR[k]) : R(0.5) * (UL[k] + UR[k]) - pf * (UL[k] - UR[k]);
       dUce[bL + k * F] = uc - UL[k];
       if (pr >= 0) dUce[bR + k * F] = uc - UR[k];
     }
   } else {
-    const double dAL = dAe[pl];
+    const R dAL = R(dAe[pl]);
 #pragma unroll
     for (int k = 0; k < N; k++) Fne[bL + k * F] = fn[k] * dAL;
     if (pr >= 0) {
-      const double dAR = dAe[pr];
+      const R dAR = R(dAe[pr]);
 #pragma unroll
       for (int k = 0; k < N; k++) Fne[bR + k * F] = -fn[k] * dAR;
     }
@@ -186,39 +211,39 @@ __global__ __launch_bounds__(256) void k_face_inv(int nFF, const int* __restrict
 // Volume kernel: [corrected gradient], flux, divergence, normal flux at faces
 // [and gradient at faces].  One thread per solution point, EB elements per
 // block, element data staged in LDS.
-template <int D, int P, bool VISC>
-__global__ __launch_bounds__(256) void k_volume(int E, const double* __restrict__ Ue, const double* __restrict__ dUce,
+template <int D, int P, bool VISC, class R>
+__global__ __launch_bounds__(256) void k_volume(int E, const double* __restrict__ Ue, const R* __restrict__ dUce,
                                                  const double* __restrict__ JGe, const double* __restrict__ detJe,
-                                                 double* __restrict__ divFe, double* __restrict__ disFne,
-                                                 double* __restrict__ dUfe, Tabs tb, Phys ph) {
+                                                 R* __restrict__ divFe, R* __restrict__ disFne, R* __restrict__ dUfe,
+                                                 Tabs tb, Phys ph) {
   using Sh = Shape<D, P>;
   constexpr int S = Sh::S, F = Sh::F, N = Sh::N, EB = Sh::EB, P1 = Sh::P1, NC = Sh::NC;
-  __shared__ double sU[EB * N * S];
-  __shared__ double sF[EB * D * N * S];
-  __shared__ double sG[VISC ? EB * D * N * S : 1];
+  __shared__ R sU[EB * N * S];
+  __shared__ R sF[EB * D * N * S];
+  __shared__ R sG[VISC ? EB * D * N * S : 1];
   const int e0 = blockIdx.x * EB, nb = min(EB, E - e0);
   {
     const double* src = Ue + (long long)e0 * N * S;
-    for (int i = threadIdx.x; i < nb * N * S; i += blockDim.x) sU[i] = src[i];
+    for (int i = threadIdx.x; i < nb * N * S; i += blockDim.x) sU[i] = R(src[i]);
   }
   __syncthreads();
   const int b = threadIdx.x / S, s = threadIdx.x - b * S;
   const bool act = b < nb;
   const long long e = e0 + b;
   if (act) {
-    double u[N];
+    R u[N];
 #pragma unroll
     for (int k = 0; k < N; k++) u[k] = sU[(b * N + k) * S + s];
-    double G[D][D];
+    R G[D][D];
 #pragma unroll
     for (int r = 0; r < D; r++)
 #pragma unroll
-      for (int x = 0; x < D; x++) G[r][x] = JGe[((e * D + r) * D + x) * S + s];
-    double Fl[D][N];
-    inviscid_flux<D, double>(u, Fl, ph.gamma);
+      for (int x = 0; x < D; x++) G[r][x] = R(JGe[((e * D + r) * D + x) * S + s]);
+    R Fl[D][N];
+    inviscid_flux<D, R>(u, Fl, R(ph.gamma));
     if constexpr (VISC) {
       // reference-space gradient (opp_grad_spts) plus correction (opp_correctU)
-      double gr[D][N];
+      R gr[D][N];
 #pragma unroll
       for (int d = 0; d < D; d++) {
         const int* gi = tb.grd_idx + (d * S + s) * P1;
@@ -226,23 +251,23 @@ __global__ __launch_bounds__(256) void k_volume(int E, const double* __restrict_
         const double* cw = tb.cu_w + (d * S + s) * NC;
 #pragma unroll
         for (int k = 0; k < N; k++) {
-          double a = 0.;
+          R a = 0;
 #pragma unroll
-          for (int m = 0; m < P1; m++) a += gw[m] * sU[(b * N + k) * S + gi[m]];
-          double c = 0.;
+          for (int m = 0; m < P1; m++) a += R(gw[m]) * sU[(b * N + k) * S + gi[m]];
+          R c = 0;
 #pragma unroll
-          for (int j = 0; j < NC; j++) c += cw[j] * dUce[(e * N + k) * F + tb.cor_idx[s * NC + j]];
+          for (int j = 0; j < NC; j++) c += R(cw[j]) * dUce[(e * N + k) * F + tb.cor_idx[s * NC + j]];
           gr[d][k] = c + a;
         }
       }
       // physical gradient (correctGradU transform, src/solver.cpp:1105-1136)
-      const double inv = 1. / detJe[e * S + s];
-      double g[D][N];
+      const R inv = R(1. / detJe[e * S + s]);
+      R g[D][N];
 #pragma unroll
       for (int k = 0; k < N; k++)
 #pragma unroll
         for (int x = 0; x < D; x++) {
-          double acc = gr[0][k] * G[0][x] + gr[1][k] * G[1][x];
+          R acc = gr[0][k] * G[0][x] + gr[1][k] * G[1][x];
           if constexpr (D == 3) acc += gr[2][k] * G[2][x];
           g[x][k] = inv * acc;
         }
@@ -250,8 +275,8 @@ __global__ __launch_bounds__(256) void k_volume(int E, const double* __restrict_
       for (int x = 0; x < D; x++)
 #pragma unroll
         for (int k = 0; k < N; k++) sG[((b * D + x) * N + k) * S + s] = g[x][k];
-      double Fv[D][N];
-      viscous_flux<D, double>(u, g, Fv, ph);
+      R Fv[D][N];
+      viscous_flux<D, R>(u, g, Fv, ph);
 #pragma unroll
       for (int d = 0; d < D; d++)
 #pragma unroll
@@ -262,7 +287,7 @@ __global__ __launch_bounds__(256) void k_volume(int E, const double* __restrict_
     for (int d1 = 0; d1 < D; d1++)
 #pragma unroll
       for (int k = 0; k < N; k++) {
-        double acc = 0.;
+        R acc = 0;
 #pragma unroll
         for (int d2 = 0; d2 < D; d2++) acc += G[d1][d2] * Fl[d2][k];
         sF[((b * D + d1) * N + k) * S + s] = acc;
@@ -273,14 +298,14 @@ __global__ __launch_bounds__(256) void k_volume(int E, const double* __restrict_
     // divergence (calcDivF_spts): sum over directions in order
 #pragma unroll
     for (int k = 0; k < N; k++) {
-      double tot = 0.;
+      R tot = 0;
 #pragma unroll
       for (int d = 0; d < D; d++) {
         const int* gi = tb.grd_idx + (d * S + s) * P1;
         const double* gw = tb.grd_w + (d * S + s) * P1;
-        double a = 0.;
+        R a = 0;
 #pragma unroll
-        for (int m = 0; m < P1; m++) a += gw[m] * sF[((b * D + d) * N + k) * S + gi[m]];
+        for (int m = 0; m < P1; m++) a += R(gw[m]) * sF[((b * D + d) * N + k) * S + gi[m]];
         tot = (d == 0) ? a : a + tot;
       }
       divFe[(e * N + k) * S + s] = tot;
@@ -293,13 +318,13 @@ __global__ __launch_bounds__(256) void k_volume(int E, const double* __restrict_
     const int* xi = tb.ext_idx + f * P1;
 #pragma unroll
     for (int k = 0; k < N; k++) {
-      double tot = 0.;
+      R tot = 0;
 #pragma unroll
       for (int d = 0; d < D; d++) {
         const double* ew = tb.efn_w + (f * D + d) * P1;
-        double a = 0.;
+        R a = 0;
 #pragma unroll
-        for (int m = 0; m < P1; m++) a += ew[m] * sF[((bb * D + d) * N + k) * S + xi[m]];
+        for (int m = 0; m < P1; m++) a += R(ew[m]) * sF[((bb * D + d) * N + k) * S + xi[m]];
         tot = (d == 0) ? a : a + tot;
       }
       disFne[(ee * N + k) * F + f] = tot;
@@ -310,9 +335,9 @@ __global__ __launch_bounds__(256) void k_volume(int E, const double* __restrict_
       for (int x = 0; x < D; x++)
 #pragma unroll
         for (int k = 0; k < N; k++) {
-          double a = 0.;
+          R a = 0;
 #pragma unroll
-          for (int m = 0; m < P1; m++) a += xw[m] * sG[((bb * D + x) * N + k) * S + xi[m]];
+          for (int m = 0; m < P1; m++) a += R(xw[m]) * sG[((bb * D + x) * N + k) * S + xi[m]];
           dUfe[((ee * D + x) * N + k) * F + f] = a;
         }
     }
@@ -321,12 +346,12 @@ __global__ __launch_bounds__(256) void k_volume(int E, const double* __restrict_
 
 // ---------------------------------------------------------------------------
 // LDG common viscous flux at face flux points.
-template <int D, int P>
+template <int D, int P, class R>
 __global__ __launch_bounds__(256) void k_face_visc(int nFF, const int* __restrict__ pLa, const int* __restrict__ pRa,
-                                                    const int* __restrict__ bca, const double* __restrict__ Ufe,
-                                                    const double* __restrict__ dUfe, const double* __restrict__ norme,
-                                                    const double* __restrict__ dAe, double* __restrict__ Fne,
-                                                    const double* __restrict__ FnInv, Phys ph) {
+                                                    const int* __restrict__ bca, const R* __restrict__ Ufe,
+                                                    const R* __restrict__ dUfe, const double* __restrict__ norme,
+                                                    const double* __restrict__ dAe, R* __restrict__ Fne,
+                                                    const R* __restrict__ FnInv, Phys ph) {
   using Sh = Shape<D, P>;
   constexpr int F = Sh::F, N = Sh::N;
   int t = blockIdx.x * blockDim.x + threadIdx.x;
@@ -334,11 +359,11 @@ __global__ __launch_bounds__(256) void k_face_visc(int nFF, const int* __restric
   const int pl = pLa[t], pr = pRa[t], bc = bca[t];
   const long long eL = pl / F, fL = pl - eL * F;
   const long long bL = eL * N * F + fL;
-  double UL[N], UR[N], n[D], gL[D][N], gR[D][N], fn[N];
+  R UL[N], UR[N], n[D], gL[D][N], gR[D][N], fn[N];
 #pragma unroll
   for (int k = 0; k < N; k++) UL[k] = Ufe[bL + k * F];
 #pragma unroll
-  for (int d = 0; d < D; d++) n[d] = norme[(eL * D + d) * F + fL];
+  for (int d = 0; d < D; d++) n[d] = R(norme[(eL * D + d) * F + fL]);
 #pragma unroll
   for (int d = 0; d < D; d++)
 #pragma unroll
@@ -346,13 +371,13 @@ __global__ __launch_bounds__(256) void k_face_visc(int nFF, const int* __restric
   int bnd = 0;
   long long bR = 0;
   if (pr < 0) {
-    apply_bc<D, double>(bc, UL, n, UR, ph);
+    apply_bc<D, R>(bc, UL, n, UR, ph);
     bnd = (bc == 10) ? 2 : 1;
 #pragma unroll
     for (int d = 0; d < D; d++)
 #pragma unroll
       for (int k = 0; k < N; k++) gR[d][k] = gL[d][k];
-    if (bnd == 2) adiabatic_grad<D, double>(UL, UR, gR, ph.gamma);
+    if (bnd == 2) adiabatic_grad<D, R>(UL, UR, gR, R(ph.gamma));
   } else {
     const long long eR = pr / F, fR = pr - eR * F;
     bR = eR * N * F + fR;
@@ -365,12 +390,12 @@ __global__ __launch_bounds__(256) void k_face_visc(int nFF, const int* __restric
   }
 #pragma unroll
   for (int k = 0; k < N; k++) fn[k] = FnInv[(long long)k * nFF + t];
-  ldg_flux<D, double>(bnd, UL, UR, gL, gR, n, fn, ph);
-  const double dAL = dAe[pl];
+  ldg_flux<D, R>(bnd, UL, UR, gL, gR, n, fn, ph);
+  const R dAL = R(dAe[pl]);
 #pragma unroll
   for (int k = 0; k < N; k++) Fne[bL + k * F] = fn[k] * dAL;
   if (pr >= 0) {
-    const double dAR = dAe[pr];
+    const R dAR = R(dAe[pr]);
 #pragma unroll
     for (int k = 0; k < N; k++) Fne[bR + k * F] = -fn[k] * dAR;
   }
@@ -378,19 +403,19 @@ __global__ __launch_bounds__(256) void k_face_visc(int nFF, const int* __restric
 
 // ---------------------------------------------------------------------------
 // Correction of the divergence (correctDivFlux), RK update (timeStepA/B
-// folded into one pass per stage) and U at the flux points for the next stage.
-// mode: 0 first classical stage, 1 middle classical stage, 2 last classical
-// stage, 3 single-stage forward Euler, 4 Jameson-form stage.
-template <int D, int P>
-__global__ __launch_bounds__(256) void k_rk(int E, int mode, double a, double bcoef, const double* __restrict__ divFe,
-                                             const double* __restrict__ Fne, const double* __restrict__ disFne,
+// folded into one pass per stage, FP64) and U at the flux points for the
+// next stage.  mode: 0 first classical stage, 1 middle classical stage,
+// 2 last classical stage, 3 single-stage forward Euler, 4 Jameson-form stage.
+template <int D, int P, class R>
+__global__ __launch_bounds__(256) void k_rk(int E, int mode, double a, double bcoef, const R* __restrict__ divFe,
+                                             const R* __restrict__ Fne, const R* __restrict__ disFne,
                                              const double* __restrict__ detJe, const double* __restrict__ dtp,
                                              const double* __restrict__ eleDt, int local, double* __restrict__ Ue,
                                              double* __restrict__ U0e, double* __restrict__ Uacc,
-                                             double* __restrict__ divF0e, double* __restrict__ Ufe, Tabs tb) {
+                                             double* __restrict__ divF0e, R* __restrict__ Ufe, Tabs tb) {
   using Sh = Shape<D, P>;
   constexpr int S = Sh::S, F = Sh::F, N = Sh::N, EB = Sh::EB, P1 = Sh::P1, NC = Sh::NC;
-  __shared__ double sU[EB * N * S];
+  __shared__ R sU[EB * N * S];
   const int e0 = blockIdx.x * EB, nb = min(EB, E - e0);
   const int b = threadIdx.x / S, s = threadIdx.x - b * S;
   if (b < nb) {
@@ -400,13 +425,13 @@ __global__ __launch_bounds__(256) void k_rk(int E, int mode, double a, double bc
 #pragma unroll
     for (int k = 0; k < N; k++) {
       const long long iu = (e * N + k) * S + s;
-      double c = 0.;
+      R c = 0;
 #pragma unroll
       for (int j = 0; j < NC; j++) {
         const long long jf = (e * N + k) * F + tb.cor_idx[s * NC + j];
-        c += tb.cor_w[s * NC + j] * (Fne[jf] - disFne[jf]);
+        c += R(tb.cor_w[s * NC + j]) * (Fne[jf] - disFne[jf]);
       }
-      const double r = c + divFe[iu];
+      const double r = double(c + divFe[iu]);
       if (divF0e) divF0e[iu] = r;
       const double u = Ue[iu];
       double un;
@@ -425,7 +450,7 @@ __global__ __launch_bounds__(256) void k_rk(int E, int mode, double a, double bc
         un = U0e[iu] - a * r / dj * dt;
       }
       Ue[iu] = un;
-      sU[(b * N + k) * S + s] = un;
+      sU[(b * N + k) * S + s] = R(un);
     }
   }
   __syncthreads();
@@ -433,24 +458,22 @@ __global__ __launch_bounds__(256) void k_rk(int E, int mode, double a, double bc
     int bb = i / F, f = i - bb * F;
 #pragma unroll
     for (int k = 0; k < N; k++) {
-      double acc = 0.;
+      R acc = 0;
 #pragma unroll
-      for (int m = 0; m < P1; m++) acc += tb.ext_w[f * P1 + m] * sU[(bb * N + k) * S + tb.ext_idx[f * P1 + m]];
+      for (int m = 0; m < P1; m++) acc += R(tb.ext_w[f * P1 + m]) * sU[(bb * N + k) * S + tb.ext_idx[f * P1 + m]];
       Ufe[((long long)(e0 + bb) * N + k) * F + f] = acc;
     }
   }
 }
 
-// Jameson stage 0 needs U0 = U before the update; the copy happens in k_rk
-// (mode 4 reads U0e), so a small copy kernel runs first for that form.
 __global__ void k_copy_em(long long n, double* __restrict__ dst, const double* __restrict__ src) {
   long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x;
   if (i < n) dst[i] = src[i];
 }
 
 // Wave speeds from U at the flux points (ele::calcWaveSpFpts), element-major.
-template <int D, int P>
-__global__ void k_wave_em(int E, const double* __restrict__ Ufe, const double* __restrict__ norme,
+template <int D, int P, class R>
+__global__ void k_wave_em(int E, const R* __restrict__ Ufe, const double* __restrict__ norme,
                           double* __restrict__ waveSpe, double gamma) {
   using Sh = Shape<D, P>;
   constexpr int F = Sh::F, N = Sh::N;
@@ -459,7 +482,7 @@ __global__ void k_wave_em(int E, const double* __restrict__ Ufe, const double* _
   long long e = i / F, f = i - e * F;
   double U[N];
 #pragma unroll
-  for (int k = 0; k < N; k++) U[k] = Ufe[(e * N + k) * F + f];
+  for (int k = 0; k < N; k++) U[k] = double(Ufe[(e * N + k) * F + f]);
   double rho = U[0], u = U[1] / rho, v = U[2] / rho, w = 0;
   if constexpr (D == 3) w = U[3] / rho;
   double rhoVSq = rho * (u * u + v * v + w * w);
@@ -530,8 +553,6 @@ static bool build_tables(const Ctx& c, const fhip_desc* d, std::vector<char>& bl
   std::vector<int> ext_idx((size_t)F * P1, 0), grd_idx((size_t)D * S * P1, 0), cor_idx((size_t)S * NC, 0);
   std::vector<double> ext_w((size_t)F * P1, 0.), efn_w((size_t)F * D * P1, 0.), grd_w((size_t)D * S * P1, 0.),
       cor_w((size_t)S * NC, 0.), cu_w((size_t)D * S * NC, 0.);
-  // extrapolation rows; the same support must carry every direction of
-  // opp_extrapolate_fn
   for (int f = 0; f < F; f++) {
     int n = 0;
     for (int s = 0; s < S; s++) {
@@ -593,6 +614,23 @@ static bool build_tables(const Ctx& c, const fhip_desc* d, std::vector<char>& bl
   return true;
 }
 
+template <class R>
+static void ensure_work(Ctx& c) {
+  Work<R>& w = fs(c).work<R>();
+  if (w.ready) return;
+  const long long E = c.z.E, S = c.z.S, F = c.z.F, N = c.z.N, D = c.z.D;
+  w.divFe = dal<R>(E * N * S);
+  w.Ufe = dal<R>(E * N * F);
+  w.Fne = dal<R>(E * N * F);
+  w.disFne = dal<R>(E * N * F);
+  if (c.prm.viscous) {
+    w.dUce = dal<R>(E * N * F);
+    w.dUfe = dal<R>(E * D * N * F);
+    w.FnInv = dal<R>((long long)c.z.nFaceFpts * N);
+  }
+  w.ready = true;
+}
+
 bool fused_setup(Ctx& c, const fhip_desc* d) {
   const Sizes& z = c.z;
   if (z.P < 1 || z.P > 5) return false;
@@ -601,7 +639,6 @@ bool fused_setup(Ctx& c, const fhip_desc* d) {
   if (!build_tables(c, d, blob, off)) return false;
   auto* st = new FusedState();
   c.fused = st;
-  st->P1 = z.P + 1;
   st->tab_mem = dal<char>(blob.size());
   FCHK(hipMemcpy(st->tab_mem, blob.data(), blob.size(), hipMemcpyHostToDevice));
   char* base = (char*)st->tab_mem;
@@ -612,18 +649,8 @@ bool fused_setup(Ctx& c, const fhip_desc* d) {
   st->Ue = dal<double>(E * N * S);
   st->U0e = dal<double>(E * N * S);
   st->Uacc = dal<double>(E * N * S);
-  st->divFe = dal<double>(E * N * S);
   st->divF0e = dal<double>(E * N * S);
-  st->Ufe = dal<double>(E * N * F);
-  st->Fne = dal<double>(E * N * F);
-  st->disFne = dal<double>(E * N * F);
   st->waveSpe = dal<double>(E * F);
-  if (c.prm.viscous) {
-    st->dUce = dal<double>(E * N * F);
-    st->dUfe = dal<double>(E * D * N * F);
-    st->FnInv = dal<double>((long long)z.nFaceFpts * N);
-  }
-  // metrics, element-major
   st->JGe = dal<double>(E * D * D * S);
   st->detJe = dal<double>(E * S);
   st->norme = dal<double>(E * D * F);
@@ -647,7 +674,6 @@ bool fused_setup(Ctx& c, const fhip_desc* d) {
     FCHK(hipMemcpy(st->norme, nr.data(), nr.size() * 8, hipMemcpyHostToDevice));
     FCHK(hipMemcpy(st->dAe, da.data(), da.size() * 8, hipMemcpyHostToDevice));
   }
-  // face flux points as ele*F+fpt
   {
     std::vector<int> pl(z.nFaceFpts), pr(z.nFaceFpts), bc(z.nFaceFpts);
     for (int t = 0; t < z.nFaceFpts; t++) {
@@ -672,71 +698,76 @@ void fused_release(Ctx& c) {
   c.fused = nullptr;
 }
 
-// ---- per-order dispatch ---------------------------------------------------
+// ---- per-order / per-precision dispatch ------------------------------------
 
-template <int D, int P>
+template <int D, int P, class R>
 struct Runner {
   using Sh = Shape<D, P>;
+  static const char* nm(const char* f64, const char* f32) { return sizeof(R) == 8 ? f64 : f32; }
   static void extrap(Ctx& c) {
     FusedState& st = fs(c);
-    Launch L(c, "fused_extrapolate");
-    hipLaunchKernelGGL((k_extrap<D, P>), dim3(nblk(c.z.E, Sh::EB)), dim3(Sh::NT), 0, c.stream, c.z.E, st.Ue, st.Ufe, st.tb);
+    Work<R>& w = st.work<R>();
+    Launch L(c, nm("fused_extrapolate", "mixed_extrapolate"));
+    hipLaunchKernelGGL((k_extrap<D, P, R>), dim3(nblk(c.z.E, Sh::EB)), dim3(Sh::NT), 0, c.stream, c.z.E, st.Ue, w.Ufe, st.tb);
     FCHK(hipGetLastError());
   }
   static void wave(Ctx& c) {
     FusedState& st = fs(c);
     long long n = (long long)c.z.E * Sh::F;
-    Launch L(c, "fused_wave_speed");
-    hipLaunchKernelGGL((k_wave_em<D, P>), dim3(nblk(n, 256)), dim3(256), 0, c.stream, c.z.E, st.Ufe, st.norme, st.waveSpe,
-                       c.ph.gamma);
+    Launch L(c, nm("fused_wave_speed", "mixed_wave_speed"));
+    hipLaunchKernelGGL((k_wave_em<D, P, R>), dim3(nblk(n, 256)), dim3(256), 0, c.stream, c.z.E, st.work<R>().Ufe, st.norme,
+                       st.waveSpe, c.ph.gamma);
     FCHK(hipGetLastError());
   }
   static void dt(Ctx& c, double cflfac) {
     FusedState& st = fs(c);
     unsigned nb = nblk(c.z.E, 256);
     {
-      Launch L(c, "fused_ele_dt");
+      Launch L(c, nm("fused_ele_dt", "mixed_ele_dt"));
       hipLaunchKernelGGL((k_ele_dt_em<D, P>), dim3(nb), dim3(256), 0, c.stream, c.z.E, st.detJe, c.ws, c.wf, st.waveSpe,
                          st.dAe, cflfac, c.eleDt, c.partial);
       FCHK(hipGetLastError());
     }
-    Launch L(c, "fused_dt_min");
+    Launch L(c, nm("fused_dt_min", "mixed_dt_min"));
     hipLaunchKernelGGL(k_min_final_f, dim3(1), dim3(256), 0, c.stream, (int)nb, c.partial, c.d_dt);
     FCHK(hipGetLastError());
   }
   static void residual(Ctx& c) {
     FusedState& st = fs(c);
+    Work<R>& w = st.work<R>();
     const int nFF = c.z.nFaceFpts;
     const bool visc = c.prm.viscous != 0;
     if (nFF > 0) {
-      Launch L(c, "fused_face_inviscid");
-      hipLaunchKernelGGL((k_face_inv<D, P>), dim3(nblk(nFF, 256)), dim3(256), 0, c.stream, nFF, st.pL, st.pR, st.bc, st.Ufe,
-                         st.norme, st.dAe, st.Fne, st.FnInv, st.waveSpe, st.dUce, c.ph);
+      Launch L(c, nm("fused_face_inviscid", "mixed_face_inviscid"));
+      hipLaunchKernelGGL((k_face_inv<D, P, R>), dim3(nblk(nFF, 256)), dim3(256), 0, c.stream, nFF, st.pL, st.pR, st.bc,
+                         w.Ufe, st.norme, st.dAe, w.Fne, w.FnInv, st.waveSpe, w.dUce, c.ph);
       FCHK(hipGetLastError());
     }
     {
-      Launch L(c, visc ? "fused_volume_viscous" : "fused_volume_inviscid");
+      Launch L(c, visc ? nm("fused_volume_viscous", "mixed_volume_viscous")
+                       : nm("fused_volume_inviscid", "mixed_volume_inviscid"));
       if (visc)
-        hipLaunchKernelGGL((k_volume<D, P, true>), dim3(nblk(c.z.E, Sh::EB)), dim3(Sh::NT), 0, c.stream, c.z.E, st.Ue, st.dUce,
-                           st.JGe, st.detJe, st.divFe, st.disFne, st.dUfe, st.tb, c.ph);
+        hipLaunchKernelGGL((k_volume<D, P, true, R>), dim3(nblk(c.z.E, Sh::EB)), dim3(Sh::NT), 0, c.stream, c.z.E, st.Ue,
+                           w.dUce, st.JGe, st.detJe, w.divFe, w.disFne, w.dUfe, st.tb, c.ph);
       else
-        hipLaunchKernelGGL((k_volume<D, P, false>), dim3(nblk(c.z.E, Sh::EB)), dim3(Sh::NT), 0, c.stream, c.z.E, st.Ue,
-                           st.dUce, st.JGe, st.detJe, st.divFe, st.disFne, st.dUfe, st.tb, c.ph);
+        hipLaunchKernelGGL((k_volume<D, P, false, R>), dim3(nblk(c.z.E, Sh::EB)), dim3(Sh::NT), 0, c.stream, c.z.E, st.Ue,
+                           w.dUce, st.JGe, st.detJe, w.divFe, w.disFne, w.dUfe, st.tb, c.ph);
       FCHK(hipGetLastError());
     }
     if (visc && nFF > 0) {
-      Launch L(c, "fused_face_viscous");
-      hipLaunchKernelGGL((k_face_visc<D, P>), dim3(nblk(nFF, 256)), dim3(256), 0, c.stream, nFF, st.pL, st.pR, st.bc, st.Ufe,
-                         st.dUfe, st.norme, st.dAe, st.Fne, st.FnInv, c.ph);
+      Launch L(c, nm("fused_face_viscous", "mixed_face_viscous"));
+      hipLaunchKernelGGL((k_face_visc<D, P, R>), dim3(nblk(nFF, 256)), dim3(256), 0, c.stream, nFF, st.pL, st.pR, st.bc,
+                         w.Ufe, w.dUfe, st.norme, st.dAe, w.Fne, w.FnInv, c.ph);
       FCHK(hipGetLastError());
     }
   }
   static void rk(Ctx& c, int mode, double a, double b, bool write0) {
     FusedState& st = fs(c);
-    Launch L(c, "fused_rk_update");
-    hipLaunchKernelGGL((k_rk<D, P>), dim3(nblk(c.z.E, Sh::EB)), dim3(Sh::NT), 0, c.stream, c.z.E, mode, a, b, st.divFe, st.Fne,
-                       st.disFne, st.detJe, c.d_dt, c.eleDt, c.prm.dt_type == 2 ? 1 : 0, st.Ue, st.U0e, st.Uacc,
-                       write0 ? st.divF0e : nullptr, st.Ufe, st.tb);
+    Work<R>& w = st.work<R>();
+    Launch L(c, nm("fused_rk_update", "mixed_rk_update"));
+    hipLaunchKernelGGL((k_rk<D, P, R>), dim3(nblk(c.z.E, Sh::EB)), dim3(Sh::NT), 0, c.stream, c.z.E, mode, a, b, w.divFe,
+                       w.Fne, w.disFne, st.detJe, c.d_dt, c.eleDt, c.prm.dt_type == 2 ? 1 : 0, st.Ue, st.U0e, st.Uacc,
+                       write0 ? st.divF0e : nullptr, w.Ufe, st.tb);
     FCHK(hipGetLastError());
   }
 };
@@ -747,32 +778,37 @@ struct Ops {
   void (*dt)(Ctx&, double);
   void (*residual)(Ctx&);
   void (*rk)(Ctx&, int, double, double, bool);
+  void (*ensure)(Ctx&);
 };
 
-template <int D, int P>
+template <int D, int P, class R>
 static Ops ops_for() {
-  return Ops{Runner<D, P>::extrap, Runner<D, P>::wave, Runner<D, P>::dt, Runner<D, P>::residual, Runner<D, P>::rk};
+  using Rn = Runner<D, P, R>;
+  return Ops{Rn::extrap, Rn::wave, Rn::dt, Rn::residual, Rn::rk, ensure_work<R>};
 }
 
-static Ops get_ops(const Ctx& c) {
+template <class R>
+static Ops get_ops_r(const Ctx& c) {
   const int D = c.z.D, P = c.z.P;
   if (D == 2) {
     switch (P) {
-      case 1: return ops_for<2, 1>();
-      case 2: return ops_for<2, 2>();
-      case 3: return ops_for<2, 3>();
-      case 4: return ops_for<2, 4>();
-      default: return ops_for<2, 5>();
+      case 1: return ops_for<2, 1, R>();
+      case 2: return ops_for<2, 2, R>();
+      case 3: return ops_for<2, 3, R>();
+      case 4: return ops_for<2, 4, R>();
+      default: return ops_for<2, 5, R>();
     }
   }
   switch (P) {
-    case 1: return ops_for<3, 1>();
-    case 2: return ops_for<3, 2>();
-    case 3: return ops_for<3, 3>();
-    case 4: return ops_for<3, 4>();
-    default: return ops_for<3, 5>();
+    case 1: return ops_for<3, 1, R>();
+    case 2: return ops_for<3, 2, R>();
+    case 3: return ops_for<3, 3, R>();
+    case 4: return ops_for<3, 4, R>();
+    default: return ops_for<3, 5, R>();
   }
 }
+
+static Ops get_ops(const Ctx& c, bool mixed) { return mixed ? get_ops_r<float>(c) : get_ops_r<double>(c); }
 
 static double cfl_limit_f(int order) {  // getCFLLimit, src/global.cpp:81-105
   static const double t[6] = {1.393, 0.464, 0.235, 0.1453, 0.100, 0.0736};
@@ -780,15 +816,17 @@ static double cfl_limit_f(int order) {  // getCFLLimit, src/global.cpp:81-105
   return t[order];
 }
 
-void fused_begin(Ctx& c) {
+void fused_begin(Ctx& c, bool mixed) {
   FusedState& st = fs(c);
   const Sizes& z = c.z;
+  Ops o = get_ops(c, mixed);
+  o.ensure(c);
   long long nS = (long long)z.E * z.N * z.S;
   hipLaunchKernelGGL(k_to_em, dim3(nblk(nS, 256)), dim3(256), 0, c.stream, nS, z.S, z.E, z.N, c.U, st.Ue);
   long long nW = (long long)z.E * z.F;
   hipLaunchKernelGGL(k_to_em, dim3(nblk(nW, 256)), dim3(256), 0, c.stream, nW, z.F, z.E, 1, c.waveSp, st.waveSpe);
   FCHK(hipGetLastError());
-  get_ops(c).extrap(c);
+  o.extrap(c);
 }
 
 void fused_end(Ctx& c) {
@@ -802,9 +840,9 @@ void fused_end(Ctx& c) {
   FCHK(hipGetLastError());
 }
 
-// One iteration of solver::update (src/solver.cpp:319-363) on the fused path.
-void fused_rk_step(Ctx& c, bool refresh) {
-  Ops o = get_ops(c);
+// One iteration of solver::update (src/solver.cpp:319-363) on a fused path.
+void fused_rk_step(Ctx& c, bool refresh, bool mixed) {
+  Ops o = get_ops(c, mixed);
   FusedState& st = fs(c);
   const int nRK = c.z.nRK;
   const double* A = c.prm.rk_a;

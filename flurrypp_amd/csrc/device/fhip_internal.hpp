@@ -69,6 +69,7 @@ struct Ctx {
   // captured RK steps (fused path): index 0 without, 1 with wave-speed refresh
   hipGraphExec_t graph[2][3] = {{nullptr, nullptr, nullptr}, {nullptr, nullptr, nullptr}};
   bool fused_ok = false;
+  bool use_graphs = true;
   void* fused = nullptr;  // FusedState (fhip_fused.hip)
 };
 
@@ -113,9 +114,9 @@ int check_state(Ctx& c);
 // fused path (fhip_fused.hip)
 bool fused_setup(Ctx& c, const fhip_desc* d);  // false: configuration not covered
 void fused_release(Ctx& c);
-void fused_begin(Ctx& c);  // import U and wave speeds, extrapolate U to the faces
-void fused_end(Ctx& c);    // export U, divF_spts[0] and wave speeds
-void fused_rk_step(Ctx& c, bool refresh);
+void fused_begin(Ctx& c, bool mixed);  // import U and wave speeds, extrapolate U to the faces
+void fused_end(Ctx& c);                // export U, divF_spts[0] and wave speeds
+void fused_rk_step(Ctx& c, bool refresh, bool mixed);
 
 }  // namespace fhip
 

@@ -125,7 +125,10 @@ enum {
 enum {
   FHIP_PATH_AUTO = 0,      /* fused kernels where available, else reference-structured */
   FHIP_PATH_STAGED = 1,    /* one kernel per reference step, dense operators */
-  FHIP_PATH_FUSED = 2      /* tensor-product (sum-factorised) fused kernels */
+  FHIP_PATH_FUSED = 2,     /* tensor-product (sum-factorised) fused kernels, FP64 */
+  FHIP_PATH_MIXED = 3      /* fused kernels with FP32 fluxes and face/volume
+                              intermediates; FP64 state, RK accumulation,
+                              metric terms, wave speeds, dt and residual */
 };
 
 typedef struct fhip_ctx fhip_ctx;
@@ -165,6 +168,11 @@ int fhip_update(fhip_ctx* ctx, int n_steps, int path);
 int fhip_sync(fhip_ctx* ctx);
 /* 1 if fhip_update can take `path` for this context's configuration, else 0. */
 int fhip_path_supported(fhip_ctx* ctx, int path);
+
+/* Options: FHIP_OPT_GRAPHS (1, default on) replays the fused path's RK step
+ * as one captured HIP graph. */
+enum { FHIP_OPT_GRAPHS = 1 };
+int fhip_set_option(fhip_ctx* ctx, int option, int value);
 
 /* Residual norm over divF_spts[0] (resType 1, 2 or 3): out[nFields]. */
 int fhip_residual_norm(fhip_ctx* ctx, int res_type, double* out);

@@ -122,6 +122,46 @@ def test_trajectory(case, path):
     assert np.allclose(r_s, r_o, rtol=1e-10, atol=0)
 
 
+MIXED_TOL = 1e-6  # relative L2 of the state against the FP64 oracle
+
+
+@pytest.mark.parametrize("name", ["vortex2d_p3_jitter_rusanov", "tgv3d_p3_ns_jitter", "bump2d_p3_ns_adiabatic",
+                                  "vortex2d_p2_roe"])
+def test_mixed_precision_error(name):
+    """Mixed path (FP32 fluxes/intermediates, FP64 state and RK sums) stays
+    within 1e-6 relative L2 of the FP64 oracle after 10 RK steps."""
+    from flurrypp_amd._native import PATH_MIXED
+    setup, U = CASES[name]()
+    p = setup.params
+    s = Solver(p, setup=setup)
+    o = Oracle.from_setup(setup)
+    s.initializeSolution(U)
+    o.initializeSolution(U, p.dtType)
+    s.update(10, path=PATH_MIXED)
+    o.update(10)
+    a, b = s.get_U(), o.get_U()
+    err = rel_l2(a, b)
+    err_rho = rel_l2(a[..., 0], b[..., 0])
+    print(f"{name} mixed: U rel_l2={err:.3e} rho rel_l2={err_rho:.3e}", flush=True)
+    assert err < MIXED_TOL and err_rho < MIXED_TOL
+
+
+def test_graph_replay_matches_eager():
+    """The fused step replayed from a captured HIP graph equals the eager
+    launches bit for bit, over several replays with host work between."""
+    from flurrypp_amd import _native as nat
+    setup, U = cases.vortex2d(n=8, order=3)
+    a = Solver(setup.params, setup=setup)
+    b = Solver(setup.params, setup=setup)
+    nat.hip_lib().fhip_set_option(b._ctx, 1, 0)  # graphs off
+    for x in (a, b):
+        x.initializeSolution(U)
+    for rep in range(4):
+        a.update(3, path=PATH_FUSED)
+        b.update(3, path=PATH_FUSED)
+        assert np.array_equal(a.get_U(), b.get_U()), f"replay {rep}"
+
+
 def test_granular_sequence_matches_update():
     """calcResidual/timeStepA/timeStepB/copy* called one by one, as
     solver::update does, equal fhip_update."""
