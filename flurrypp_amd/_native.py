@@ -45,6 +45,8 @@ class FhipDesc(C.Structure):
         ("n_faces", C.c_int),
         ("face_ele_l", _ip), ("face_loc_l", _ip), ("face_ele_r", _ip), ("face_loc_r", _ip),
         ("face_rel_rot", _ip), ("face_bc", _ip),
+        ("n_meshes", C.c_int), ("mesh_ele_start", _ip),
+        ("n_bnd_states", C.c_int), ("bnd_states", _dp), ("face_bnd_state", _ip),
     ]
 
 
@@ -104,6 +106,9 @@ def hip_lib() -> C.CDLL:
         lib.fhip_path_supported.argtypes = [C.c_void_p, C.c_int]
         lib.fhip_set_option.argtypes = [C.c_void_p, C.c_int, C.c_int]
         lib.fhip_residual_norm.argtypes = [C.c_void_p, C.c_int, _dp]
+        lib.fhip_mesh_residual_norms.argtypes = [C.c_void_p, C.c_int, _dp]
+        lib.fhip_set_mesh_active.argtypes = [C.c_void_p, _ip]
+        lib.fhip_update_many.argtypes = [C.POINTER(C.c_void_p), C.c_int, C.c_int, C.c_int]
         lib.fhip_array_size.argtypes = [C.c_void_p, C.c_int]
         lib.fhip_array_size.restype = C.c_longlong
         lib.fhip_get_array.argtypes = [C.c_void_p, C.c_int, C.c_int, C.c_void_p]

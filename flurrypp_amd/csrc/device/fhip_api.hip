@@ -258,7 +258,9 @@ int fhip_create(const fhip_desc* d, fhip_ctx** out) {
       c.flag = dcopy<int>(nullptr, 1, c.stream);
       c.h_dt = p.dt;
       c.iter = p.init_iter;
+      c.batched = d->n_meshes > 1 || d->n_bnd_states > 0;
       c.fused_ok = fhip::fused_setup(c, d);
+      require(c.fused_ok || !c.batched, "batched contexts need the fused path (quads/hexes, P 1..5)");
       HCHK(hipStreamSynchronize(c.stream));
     } catch (...) {
       destroy_ctx(c);
@@ -316,6 +318,7 @@ int fhip_calc_residual(fhip_ctx* h, int step) {
   return guard([&] {
     require(h, "null context");
     require(step >= 0 && step < h->c.z.nRK, "RK step out of range");
+    require(!h->c.batched, "batched contexts run through fhip_update");
     fhip::staged_residual(h->c, step);
     return 0;
   });
@@ -404,57 +407,107 @@ int fhip_get_iter(fhip_ctx* h, int* iter) {
   return 0;
 }
 
+static bool resolve_path(fhip::Ctx& c, int path, bool& mixed) {
+  require(path == FHIP_PATH_AUTO || path == FHIP_PATH_STAGED || path == FHIP_PATH_FUSED || path == FHIP_PATH_MIXED,
+          "unknown path");
+  mixed = path == FHIP_PATH_MIXED;
+  bool fused = mixed || (path == FHIP_PATH_FUSED) || (path == FHIP_PATH_AUTO && c.fused_ok);
+  require(!fused || c.fused_ok, "the fused paths do not support this configuration");
+  require(fused || !c.batched, "batched contexts (several meshes or a boundary-state table) need a fused path");
+  return fused;
+}
+
+// One iteration of the main loop body: iter++, solver::update.
+static void update_step(fhip::Ctx& c, bool fused, bool mixed) {
+  const int nRK = c.z.nRK;
+  c.iter++;
+  bool refresh = (c.iter == c.prm.init_iter + 1);
+  if (fused) {
+    // Steady-state steps replay one captured graph; the wave-speed refresh
+    // step and timed runs launch kernel by kernel.
+    if (!refresh && !c.timing && c.use_graphs) {
+      hipGraphExec_t& ge = c.graph[0][mixed ? 1 : 0];
+      if (!ge) {
+        hipGraph_t g = nullptr;
+        HCHK(hipStreamBeginCapture(c.stream, hipStreamCaptureModeThreadLocal));
+        fhip::fused_rk_step(c, false, mixed);
+        HCHK(hipStreamEndCapture(c.stream, &g));
+        HCHK(hipGraphInstantiate(&ge, g, nullptr, nullptr, 0));
+        (void)hipGraphDestroy(g);
+      }
+      HCHK(hipGraphLaunch(ge, c.stream));
+    } else {
+      fhip::fused_rk_step(c, refresh, mixed);
+    }
+    return;
+  }
+  // solver::update (src/solver.cpp:319-363), static grid
+  for (int step = 0; step < nRK - 1; step++) {
+    if (step == 0 && c.prm.dt_type != 0) fhip::calc_dt(c, refresh);
+    if (step == 0) fhip::copy_array(c, c.U0, c.U, c.z.nS());
+    fhip::staged_residual(c, step);
+    fhip::staged_time_step_a(c, step, c.prm.rk_a[step + 1], false);
+  }
+  fhip::staged_residual(c, nRK - 1);
+  if (c.prm.time_type < 5) {
+    if (nRK > 1) fhip::copy_array(c, c.U, c.U0, c.z.nS());
+    else if (c.prm.dt_type != 0) fhip::calc_dt(c, refresh);
+    for (int step = 0; step < nRK; step++) fhip::staged_time_step_b(c, step, false);
+  } else {
+    fhip::staged_time_step_a(c, nRK - 1, c.prm.rk_a[nRK], false);
+  }
+}
+
 int fhip_update(fhip_ctx* h, int n_steps, int path) {
   return guard([&] {
     require(h, "null context");
     require(n_steps >= 0, "n_steps must be >= 0");
-    require(path == FHIP_PATH_AUTO || path == FHIP_PATH_STAGED || path == FHIP_PATH_FUSED || path == FHIP_PATH_MIXED,
-            "unknown path");
     fhip::Ctx& c = h->c;
-    const bool mixed = path == FHIP_PATH_MIXED;
-    bool fused = mixed || (path == FHIP_PATH_FUSED) || (path == FHIP_PATH_AUTO && c.fused_ok);
-    require(!fused || c.fused_ok, "the fused paths do not support this configuration");
-    const int nRK = c.z.nRK;
+    bool mixed;
+    const bool fused = resolve_path(c, path, mixed);
     if (fused && n_steps > 0) fhip::fused_begin(c, mixed);
-    for (int it = 0; it < n_steps; it++) {
-      c.iter++;
-      bool refresh = (c.iter == c.prm.init_iter + 1);
-      if (fused) {
-        // Steady-state steps replay one captured graph; the wave-speed
-        // refresh step and timed runs launch kernel by kernel.
-        if (!refresh && !c.timing && c.use_graphs) {
-          hipGraphExec_t& ge = c.graph[0][mixed ? 1 : 0];
-          if (!ge) {
-            hipGraph_t g = nullptr;
-            HCHK(hipStreamBeginCapture(c.stream, hipStreamCaptureModeThreadLocal));
-            fhip::fused_rk_step(c, false, mixed);
-            HCHK(hipStreamEndCapture(c.stream, &g));
-            HCHK(hipGraphInstantiate(&ge, g, nullptr, nullptr, 0));
-            (void)hipGraphDestroy(g);
-          }
-          HCHK(hipGraphLaunch(ge, c.stream));
-        } else {
-          fhip::fused_rk_step(c, refresh, mixed);
-        }
-        continue;
-      }
-      // solver::update (src/solver.cpp:319-363), static grid
-      for (int step = 0; step < nRK - 1; step++) {
-        if (step == 0 && c.prm.dt_type != 0) fhip::calc_dt(c, refresh);
-        if (step == 0) fhip::copy_array(c, c.U0, c.U, c.z.nS());
-        fhip::staged_residual(c, step);
-        fhip::staged_time_step_a(c, step, c.prm.rk_a[step + 1], false);
-      }
-      fhip::staged_residual(c, nRK - 1);
-      if (c.prm.time_type < 5) {
-        if (nRK > 1) fhip::copy_array(c, c.U, c.U0, c.z.nS());
-        else if (c.prm.dt_type != 0) fhip::calc_dt(c, refresh);
-        for (int step = 0; step < nRK; step++) fhip::staged_time_step_b(c, step, false);
-      } else {
-        fhip::staged_time_step_a(c, nRK - 1, c.prm.rk_a[nRK], false);
-      }
-    }
+    for (int it = 0; it < n_steps; it++) update_step(c, fused, mixed);
     if (fused && n_steps > 0) fhip::fused_end(c);
+    return 0;
+  });
+}
+
+int fhip_update_many(fhip_ctx** hs, int n, int n_steps, int path) {
+  return guard([&] {
+    require(hs && n >= 0 && n_steps >= 0, "bad arguments");
+    std::vector<bool> fused(n), mixed(n);
+    for (int i = 0; i < n; i++) {
+      require(hs[i] != nullptr, "null context");
+      bool m;
+      fused[i] = resolve_path(hs[i]->c, path, m);
+      mixed[i] = m;
+    }
+    if (n_steps == 0) return 0;
+    for (int i = 0; i < n; i++)
+      if (fused[i]) fhip::fused_begin(hs[i]->c, mixed[i]);
+    for (int it = 0; it < n_steps; it++)
+      for (int i = 0; i < n; i++) update_step(hs[i]->c, fused[i], mixed[i]);
+    for (int i = 0; i < n; i++)
+      if (fused[i]) fhip::fused_end(hs[i]->c);
+    return 0;
+  });
+}
+
+int fhip_mesh_residual_norms(fhip_ctx* h, int res_type, double* out) {
+  return guard([&] {
+    require(h && out, "null argument");
+    require(res_type >= 1 && res_type <= 3, "resType must be 1, 2 or 3");
+    require(h->c.fused_ok, "per-mesh residuals need the fused path");
+    fhip::fused_mesh_residuals(h->c, res_type, out);
+    return 0;
+  });
+}
+
+int fhip_set_mesh_active(fhip_ctx* h, const int* active) {
+  return guard([&] {
+    require(h && active, "null argument");
+    require(h->c.fused_ok, "mesh activity flags need the fused path");
+    fhip::fused_set_active(h->c, active);
     return 0;
   });
 }

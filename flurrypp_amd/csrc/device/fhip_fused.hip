@@ -89,12 +89,18 @@ struct FusedState {
   double *Ue = nullptr, *U0e = nullptr, *Uacc = nullptr, *divF0e = nullptr, *JGe = nullptr, *detJe = nullptr,
          *norme = nullptr, *dAe = nullptr, *waveSpe = nullptr;
   int *pL = nullptr, *pR = nullptr, *bc = nullptr;
+  // batching: boundary-state table, element -> mesh map, per-mesh dt and activity
+  int nMesh = 1;
+  std::vector<int> h_meshStart;
+  double *bst = nullptr, *meshDt = nullptr, *meshRes = nullptr;
+  int *eleMesh = nullptr, *meshStart = nullptr, *active = nullptr;
   Work<double> w64;
   Work<float> w32;
   ~FusedState() {
     w64.release();
     w32.release();
-    void* ps[] = {tab_mem, Ue, U0e, Uacc, divF0e, JGe, detJe, norme, dAe, waveSpe, pL, pR, bc};
+    void* ps[] = {tab_mem, Ue,  U0e,    Uacc,   divF0e,    JGe,    detJe, norme, dAe, waveSpe,
+                  pL,      pR,  bc,     bst,    meshDt,    meshRes, eleMesh, meshStart, active};
     for (void* p : ps)
       if (p) (void)hipFree(p);
   }
@@ -154,12 +160,32 @@ __global__ __launch_bounds__(256) void k_extrap(int E, const double* __restrict_
 
 // ---------------------------------------------------------------------------
 // Common inviscid flux at face flux points; pL/pR are ele*F+fpt.
+// Boundary state of a face flux point: the packed code holds the BC number in
+// its low 4 bits and (row of the boundary-state table + 1) above them.
+template <int D, class R>
+__device__ __forceinline__ void bc_state(int code, const R* UL, const R* n, R* UR, const Phys& ph,
+                                         const double* __restrict__ bst) {
+  const int bc = code & 15, row = (code >> 4) - 1;
+  if (row < 0) {
+    apply_bc<D, R>(bc, UL, n, UR, ph);
+  } else {
+    Phys q = ph;
+    q.rho_bound = bst[row * 5 + 0];
+    q.u_bound = bst[row * 5 + 1];
+    q.v_bound = bst[row * 5 + 2];
+    q.w_bound = bst[row * 5 + 3];
+    q.p_bound = bst[row * 5 + 4];
+    apply_bc<D, R>(bc, UL, n, UR, q);
+  }
+}
+
 template <int D, int P, class R>
 __global__ __launch_bounds__(256) void k_face_inv(int nFF, const int* __restrict__ pLa, const int* __restrict__ pRa,
                                                    const int* __restrict__ bca, const R* __restrict__ Ufe,
                                                    const double* __restrict__ norme, const double* __restrict__ dAe,
                                                    R* __restrict__ Fne, R* __restrict__ FnInv,
-                                                   double* __restrict__ waveSpe, R* __restrict__ dUce, Phys ph) {
+                                                   double* __restrict__ waveSpe, R* __restrict__ dUce, Phys ph,
+                                                   const double* __restrict__ bst) {
   using Sh = Shape<D, P>;
   constexpr int F = Sh::F, N = Sh::N;
   int t = blockIdx.x * blockDim.x + threadIdx.x;
@@ -174,7 +200,7 @@ __global__ __launch_bounds__(256) void k_face_inv(int nFF, const int* __restrict
   for (int d = 0; d < D; d++) n[d] = R(norme[(eL * D + d) * F + fL]);
   long long bR = 0;
   if (pr < 0) {
-    apply_bc<D, R>(bc, UL, n, UR, ph);
+    bc_state<D, R>(bc, UL, n, UR, ph, bst);
   } else {
     const long long eR = pr / F, fR = pr - eR * F;
     bR = eR * N * F + fR;
@@ -351,7 +377,8 @@ __global__ __launch_bounds__(256) void k_face_visc(int nFF, const int* __restric
                                                     const int* __restrict__ bca, const R* __restrict__ Ufe,
                                                     const R* __restrict__ dUfe, const double* __restrict__ norme,
                                                     const double* __restrict__ dAe, R* __restrict__ Fne,
-                                                    const R* __restrict__ FnInv, Phys ph) {
+                                                    const R* __restrict__ FnInv, Phys ph,
+                                                    const double* __restrict__ bst) {
   using Sh = Shape<D, P>;
   constexpr int F = Sh::F, N = Sh::N;
   int t = blockIdx.x * blockDim.x + threadIdx.x;
@@ -371,8 +398,8 @@ __global__ __launch_bounds__(256) void k_face_visc(int nFF, const int* __restric
   int bnd = 0;
   long long bR = 0;
   if (pr < 0) {
-    apply_bc<D, R>(bc, UL, n, UR, ph);
-    bnd = (bc == 10) ? 2 : 1;
+    bc_state<D, R>(bc, UL, n, UR, ph, bst);
+    bnd = ((bc & 15) == 10) ? 2 : 1;
 #pragma unroll
     for (int d = 0; d < D; d++)
 #pragma unroll
@@ -412,7 +439,9 @@ __global__ __launch_bounds__(256) void k_rk(int E, int mode, double a, double bc
                                              const double* __restrict__ detJe, const double* __restrict__ dtp,
                                              const double* __restrict__ eleDt, int local, double* __restrict__ Ue,
                                              double* __restrict__ U0e, double* __restrict__ Uacc,
-                                             double* __restrict__ divF0e, R* __restrict__ Ufe, Tabs tb) {
+                                             double* __restrict__ divF0e, R* __restrict__ Ufe, Tabs tb,
+                                             const int* __restrict__ eleMesh, const double* __restrict__ meshDt,
+                                             const int* __restrict__ active) {
   using Sh = Shape<D, P>;
   constexpr int S = Sh::S, F = Sh::F, N = Sh::N, EB = Sh::EB, P1 = Sh::P1, NC = Sh::NC;
   __shared__ R sU[EB * N * S];
@@ -420,7 +449,9 @@ __global__ __launch_bounds__(256) void k_rk(int E, int mode, double a, double bc
   const int b = threadIdx.x / S, s = threadIdx.x - b * S;
   if (b < nb) {
     const long long e = e0 + b;
-    const double dt = local ? eleDt[e] : *dtp;
+    // dt: 0 global, 1 per element (local CFL), 2 per mesh (batched global CFL)
+    const double dt = (local == 1) ? eleDt[e] : (local == 2 ? meshDt[eleMesh[e]] : *dtp);
+    const bool frozen = active && !active[eleMesh[e]];
     const double dj = detJe[e * S + s];
 #pragma unroll
     for (int k = 0; k < N; k++) {
@@ -449,6 +480,7 @@ __global__ __launch_bounds__(256) void k_rk(int E, int mode, double a, double bc
       } else {
         un = U0e[iu] - a * r / dj * dt;
       }
+      if (frozen) un = u;
       Ue[iu] = un;
       sU[(b * N + k) * S + s] = R(un);
     }
@@ -532,6 +564,50 @@ __global__ void k_min_final_f(int n, const double* __restrict__ part, double* __
     __syncthreads();
   }
   if (threadIdx.x == 0) *out = red[0];
+}
+
+// Per-mesh minimum of the element time steps (batched contexts, dt_type 1).
+__global__ void k_mesh_dt_min(const int* __restrict__ start, const double* __restrict__ eleDt,
+                              double* __restrict__ meshDt) {
+  __shared__ double red[256];
+  const int m = blockIdx.x;
+  double v = INFINITY;
+  for (int e = start[m] + threadIdx.x; e < start[m + 1]; e += blockDim.x) v = fmin(v, eleDt[e]);
+  red[threadIdx.x] = v;
+  __syncthreads();
+  for (int st = blockDim.x / 2; st > 0; st >>= 1) {
+    if (threadIdx.x < st) red[threadIdx.x] = fmin(red[threadIdx.x], red[threadIdx.x + st]);
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) meshDt[m] = red[0];
+}
+
+// Per-mesh residual norm over divF_spts[0] (element-major), as
+// ele::getNormResidual summed over the mesh's elements (src/ele.cpp:1922-1945).
+__global__ void k_mesh_res(int S, int N, int type, const int* __restrict__ start, const double* __restrict__ divF0e,
+                           const double* __restrict__ detJe, const double* __restrict__ ws, double* __restrict__ out) {
+  __shared__ double red[256];
+  const int m = blockIdx.x;
+  for (int k = 0; k < N; k++) {
+    double r = 0;
+    for (long long i = (long long)start[m] * S + threadIdx.x; i < (long long)start[m + 1] * S; i += blockDim.x) {
+      long long e = i / S;
+      int s = (int)(i - e * S);
+      double v = divF0e[(e * N + k) * S + s], dj = detJe[i];
+      if (type == 1) r += fabs(v) * ws[s];
+      else if (type == 2) r += v * v / dj * ws[s];
+      else r = fmax(r, fabs(v) / dj);
+    }
+    red[threadIdx.x] = r;
+    __syncthreads();
+    for (int st = blockDim.x / 2; st > 0; st >>= 1) {
+      if (threadIdx.x < st)
+        red[threadIdx.x] = (type == 3) ? fmax(red[threadIdx.x], red[threadIdx.x + st]) : red[threadIdx.x] + red[threadIdx.x + st];
+      __syncthreads();
+    }
+    if (threadIdx.x == 0) out[(long long)m * N + k] = (type == 2) ? sqrt(red[0]) : red[0];
+    __syncthreads();
+  }
 }
 
 // ---------------------------------------------------------------------------
@@ -674,6 +750,32 @@ bool fused_setup(Ctx& c, const fhip_desc* d) {
     FCHK(hipMemcpy(st->norme, nr.data(), nr.size() * 8, hipMemcpyHostToDevice));
     FCHK(hipMemcpy(st->dAe, da.data(), da.size() * 8, hipMemcpyHostToDevice));
   }
+  // batching: meshes, boundary-state table
+  if (d->n_meshes > 1 || d->mesh_ele_start) {
+    int nm = std::max(1, d->n_meshes);
+    if (!d->mesh_ele_start) throw std::invalid_argument("n_meshes > 1 needs mesh_ele_start");
+    st->h_meshStart.assign(d->mesh_ele_start, d->mesh_ele_start + nm + 1);
+    if (st->h_meshStart[0] != 0 || st->h_meshStart[nm] != z.E) throw std::invalid_argument("mesh_ele_start must span [0, nEles]");
+    std::vector<int> em(z.E);
+    for (int m = 0; m < nm; m++) {
+      if (st->h_meshStart[m + 1] <= st->h_meshStart[m]) throw std::invalid_argument("mesh_ele_start must increase");
+      for (int e = st->h_meshStart[m]; e < st->h_meshStart[m + 1]; e++) em[e] = m;
+    }
+    st->nMesh = nm;
+    st->eleMesh = dal<int>(z.E);
+    st->meshStart = dal<int>(nm + 1);
+    st->meshDt = dal<double>(nm);
+    st->meshRes = dal<double>((size_t)nm * z.N);
+    FCHK(hipMemcpy(st->eleMesh, em.data(), em.size() * 4, hipMemcpyHostToDevice));
+    FCHK(hipMemcpy(st->meshStart, st->h_meshStart.data(), (nm + 1) * 4, hipMemcpyHostToDevice));
+  } else {
+    st->h_meshStart = {0, z.E};
+  }
+  if (d->n_bnd_states > 0) {
+    if (!d->bnd_states || !d->face_bnd_state) throw std::invalid_argument("boundary-state table needs bnd_states and face_bnd_state");
+    st->bst = dal<double>((size_t)d->n_bnd_states * 5);
+    FCHK(hipMemcpy(st->bst, d->bnd_states, (size_t)d->n_bnd_states * 5 * 8, hipMemcpyHostToDevice));
+  }
   {
     std::vector<int> pl(z.nFaceFpts), pr(z.nFaceFpts), bc(z.nFaceFpts);
     for (int t = 0; t < z.nFaceFpts; t++) {
@@ -681,7 +783,12 @@ bool fused_setup(Ctx& c, const fhip_desc* d) {
       pl[t] = (a % z.E) * z.F + a / z.E;
       int r = c.h_ptR[t];
       pr[t] = (r < 0) ? -1 : (r % z.E) * z.F + r / z.E;
-      bc[t] = (r < 0) ? d->face_bc[t / z.Ff] : -1;
+      int row = -1;
+      if (r < 0 && d->n_bnd_states > 0) {
+        row = d->face_bnd_state[t / z.Ff];
+        if (row < -1 || row >= d->n_bnd_states) throw std::invalid_argument("face_bnd_state out of range");
+      }
+      bc[t] = (r < 0) ? (d->face_bc[t / z.Ff] | ((row + 1) << 4)) : -1;
     }
     st->pL = dal<int>(pl.size());
     st->pR = dal<int>(pr.size());
@@ -691,6 +798,43 @@ bool fused_setup(Ctx& c, const fhip_desc* d) {
     FCHK(hipMemcpy(st->bc, bc.data(), bc.size() * 4, hipMemcpyHostToDevice));
   }
   return true;
+}
+
+void fused_mesh_residuals(Ctx& c, int type, double* out) {
+  FusedState& st = fs(c);
+  const int nm = st.nMesh;
+  int* start = st.meshStart;
+  double* res = st.meshRes;
+  int* tmpStart = nullptr;
+  double* tmpRes = nullptr;
+  if (nm == 1) {  // single mesh: a two-entry range
+    tmpStart = dal<int>(2);
+    tmpRes = dal<double>(c.z.N);
+    int h[2] = {0, c.z.E};
+    FCHK(hipMemcpyAsync(tmpStart, h, 8, hipMemcpyHostToDevice, c.stream));
+    start = tmpStart;
+    res = tmpRes;
+  }
+  hipLaunchKernelGGL(k_mesh_res, dim3(nm), dim3(256), 0, c.stream, c.z.S, c.z.N, type, start, st.divF0e, st.detJe, c.ws, res);
+  FCHK(hipGetLastError());
+  FCHK(hipMemcpyAsync(out, res, (size_t)nm * c.z.N * 8, hipMemcpyDeviceToHost, c.stream));
+  FCHK(hipStreamSynchronize(c.stream));
+  if (tmpStart) (void)hipFree(tmpStart);
+  if (tmpRes) (void)hipFree(tmpRes);
+}
+
+int fused_n_meshes(Ctx& c) { return fs(c).nMesh; }
+
+void fused_set_active(Ctx& c, const int* active) {
+  FusedState& st = fs(c);
+  if (!st.active) st.active = dal<int>(st.nMesh);
+  if (!st.eleMesh) {  // single mesh: every element belongs to mesh 0
+    std::vector<int> em(c.z.E, 0);
+    st.eleMesh = dal<int>(c.z.E);
+    FCHK(hipMemcpy(st.eleMesh, em.data(), em.size() * 4, hipMemcpyHostToDevice));
+  }
+  FCHK(hipMemcpyAsync(st.active, active, (size_t)st.nMesh * 4, hipMemcpyHostToDevice, c.stream));
+  FCHK(hipStreamSynchronize(c.stream));
 }
 
 void fused_release(Ctx& c) {
@@ -729,7 +873,10 @@ struct Runner {
       FCHK(hipGetLastError());
     }
     Launch L(c, nm("fused_dt_min", "mixed_dt_min"));
-    hipLaunchKernelGGL(k_min_final_f, dim3(1), dim3(256), 0, c.stream, (int)nb, c.partial, c.d_dt);
+    if (st.nMesh > 1)
+      hipLaunchKernelGGL(k_mesh_dt_min, dim3(st.nMesh), dim3(256), 0, c.stream, st.meshStart, c.eleDt, st.meshDt);
+    else
+      hipLaunchKernelGGL(k_min_final_f, dim3(1), dim3(256), 0, c.stream, (int)nb, c.partial, c.d_dt);
     FCHK(hipGetLastError());
   }
   static void residual(Ctx& c) {
@@ -740,7 +887,7 @@ struct Runner {
     if (nFF > 0) {
       Launch L(c, nm("fused_face_inviscid", "mixed_face_inviscid"));
       hipLaunchKernelGGL((k_face_inv<D, P, R>), dim3(nblk(nFF, 256)), dim3(256), 0, c.stream, nFF, st.pL, st.pR, st.bc,
-                         w.Ufe, st.norme, st.dAe, w.Fne, w.FnInv, st.waveSpe, w.dUce, c.ph);
+                         w.Ufe, st.norme, st.dAe, w.Fne, w.FnInv, st.waveSpe, w.dUce, c.ph, st.bst);
       FCHK(hipGetLastError());
     }
     {
@@ -757,7 +904,7 @@ struct Runner {
     if (visc && nFF > 0) {
       Launch L(c, nm("fused_face_viscous", "mixed_face_viscous"));
       hipLaunchKernelGGL((k_face_visc<D, P, R>), dim3(nblk(nFF, 256)), dim3(256), 0, c.stream, nFF, st.pL, st.pR, st.bc,
-                         w.Ufe, w.dUfe, st.norme, st.dAe, w.Fne, w.FnInv, c.ph);
+                         w.Ufe, w.dUfe, st.norme, st.dAe, w.Fne, w.FnInv, c.ph, st.bst);
       FCHK(hipGetLastError());
     }
   }
@@ -766,8 +913,9 @@ struct Runner {
     Work<R>& w = st.work<R>();
     Launch L(c, nm("fused_rk_update", "mixed_rk_update"));
     hipLaunchKernelGGL((k_rk<D, P, R>), dim3(nblk(c.z.E, Sh::EB)), dim3(Sh::NT), 0, c.stream, c.z.E, mode, a, b, w.divFe,
-                       w.Fne, w.disFne, st.detJe, c.d_dt, c.eleDt, c.prm.dt_type == 2 ? 1 : 0, st.Ue, st.U0e, st.Uacc,
-                       write0 ? st.divF0e : nullptr, w.Ufe, st.tb);
+                       w.Fne, w.disFne, st.detJe, c.d_dt, c.eleDt,
+                       c.prm.dt_type == 2 ? 1 : (c.prm.dt_type == 1 && st.nMesh > 1 ? 2 : 0), st.Ue, st.U0e, st.Uacc,
+                       write0 ? st.divF0e : nullptr, w.Ufe, st.tb, st.eleMesh, st.meshDt, st.active);
     FCHK(hipGetLastError());
   }
 };

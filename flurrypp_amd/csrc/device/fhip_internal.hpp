@@ -70,6 +70,7 @@ struct Ctx {
   hipGraphExec_t graph[2][3] = {{nullptr, nullptr, nullptr}, {nullptr, nullptr, nullptr}};
   bool fused_ok = false;
   bool use_graphs = true;
+  bool batched = false;  // several meshes or a boundary-state table
   void* fused = nullptr;  // FusedState (fhip_fused.hip)
 };
 
@@ -117,6 +118,9 @@ void fused_release(Ctx& c);
 void fused_begin(Ctx& c, bool mixed);  // import U and wave speeds, extrapolate U to the faces
 void fused_end(Ctx& c);                // export U, divF_spts[0] and wave speeds
 void fused_rk_step(Ctx& c, bool refresh, bool mixed);
+void fused_mesh_residuals(Ctx& c, int type, double* out);  // out [nMesh][N]
+int fused_n_meshes(Ctx& c);
+void fused_set_active(Ctx& c, const int* active);
 
 }  // namespace fhip
 

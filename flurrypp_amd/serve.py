@@ -1,0 +1,175 @@
+"""Serving loop: solve requests that arrive over time, with continuous batching.
+
+Every admitted request gets its own libflurry_hip context (own stream, own
+device memory, state resident on the GPU until it finishes).  Each tick of the
+loop advances every active request by `steps_per_tick` RK steps through one
+fhip_update_many call, so the kernels of all active requests are in flight on
+the device together; finished requests are evicted (their memory released)
+and waiting requests are admitted as slots and HBM budget free up.  Requests
+are either steady (stop when the residual falls below a tolerance) or
+unsteady (a fixed number of steps), in FP64 or mixed precision.
+"""
+from __future__ import annotations
+
+import ctypes as C
+import heapq
+import time
+from dataclasses import dataclass, field
+
+import numpy as np
+
+from . import _native as nat
+from . import cases
+from .solver import Solver
+
+
+@dataclass
+class Request:
+    rid: int
+    kind: str                      # "bump2d" or "vortex3d"
+    size: tuple                    # (nx, ny) or (n,)
+    order: int
+    arrival: float                 # seconds after the start of the run
+    mach: float = 0.3
+    precision: str = "fp64"        # "fp64" or "mixed"
+    steady_tol: float | None = None
+    n_steps: int | None = None
+    max_steps: int = 200000
+    # filled in by the server
+    admitted: float | None = None
+    finished: float | None = None
+    steps: int = 0
+    residual: float | None = None
+    dof: int = 0
+    result: np.ndarray | None = None
+
+
+def build_case(r: Request):
+    if r.kind == "bump2d":
+        return cases.bump2d(nx=r.size[0], ny=r.size[1], order=r.order, mach=r.mach, seed=1000 + r.rid)
+    if r.kind == "vortex3d":
+        return cases.vortex3d(n=r.size[0], order=r.order, mach=r.mach)
+    raise ValueError(r.kind)
+
+
+def footprint_bytes(nDims: int, order: int, nEles: int, viscous: bool = False) -> int:
+    """Device bytes one context allocates (both paths' arrays, FP64 + FP32 work)."""
+    P1 = order + 1
+    S, N = P1 ** nDims, nDims + 2
+    F = 2 * nDims * P1 ** (nDims - 1)
+    per_ele = 8 * (N * S * (4 + 4 + 5) + N * F * 6 + nDims * N * S + nDims * nDims * S + S + nDims * F + 2 * F)
+    per_ele += 4 * (N * S + 3 * N * F) + 12 * F
+    if viscous:
+        per_ele += 8 * (2 * nDims * N * S + 2 * nDims * N * F) + 4 * nDims * N * F
+    return int(per_ele * nEles * 1.1) + (4 << 20)
+
+
+@dataclass
+class Server:
+    hbm_budget: float = 64e9       # bytes this server may hold on the device
+    max_active: int = 64           # concurrent requests (slots)
+    steps_per_tick: int = 10
+    check_every: int = 50          # steady requests: residual check interval (steps)
+    keep_results: bool = False
+    log: object = None
+    active: list = field(default_factory=list)
+    done: list = field(default_factory=list)
+
+    def run(self, requests: list[Request]) -> dict:
+        pending = sorted(requests, key=lambda r: r.arrival)
+        waiting: list[Request] = []
+        used = 0.0
+        t0 = time.perf_counter()
+        now = lambda: time.perf_counter() - t0  # noqa: E731
+        ctx = {}
+        total_dof_stages = 0
+        while pending or waiting or self.active:
+            t = now()
+            while pending and pending[0].arrival <= t:
+                waiting.append(pending.pop(0))
+            # admission: first come, first served, within slots and HBM budget
+            while waiting and len(self.active) < self.max_active:
+                r = waiting[0]
+                setup, U = build_case(r)
+                need = footprint_bytes(setup.nDims, setup.order, setup.nEles, bool(setup.params.viscous))
+                if used + need > self.hbm_budget and self.active:
+                    break
+                waiting.pop(0)
+                s = Solver(setup.params, setup=setup)
+                s.initializeSolution(U)
+                r.admitted, r.dof = now(), setup.nEles * setup.nSpts
+                ctx[r.rid] = (s, need, setup.params.nRKSteps)
+                used += need
+                self.active.append(r)
+            if not self.active:
+                if pending:
+                    time.sleep(max(0.0, min(0.01, pending[0].arrival - now())))
+                continue
+            # one tick: every active request advances together
+            for path in (nat.PATH_FUSED, nat.PATH_MIXED):
+                group = [r for r in self.active if (r.precision == "mixed") == (path == nat.PATH_MIXED)]
+                if not group:
+                    continue
+                arr = (C.c_void_p * len(group))(*[ctx[r.rid][0]._ctx.value for r in group])
+                nat.hip_check(nat.hip_lib().fhip_update_many(arr, len(group), self.steps_per_tick, path))
+            for r in self.active:
+                ctx[r.rid][0].sync()
+                r.steps += self.steps_per_tick
+                total_dof_stages += r.dof * ctx[r.rid][2] * self.steps_per_tick
+            # completion and eviction
+            still = []
+            for r in self.active:
+                s, need, _ = ctx[r.rid]
+                fin = False
+                if r.steady_tol is not None:
+                    if r.steps % self.check_every == 0 or r.steps >= r.max_steps:
+                        res = s.residual_norm(2)
+                        r.residual = float(res[0])
+                        if not np.isfinite(res).all():
+                            raise RuntimeError(f"request {r.rid}: non-finite residual")
+                        fin = r.residual <= r.steady_tol or r.steps >= r.max_steps
+                else:
+                    fin = r.steps >= r.n_steps
+                if fin:
+                    r.finished = now()
+                    if self.keep_results:
+                        r.result = s.get_U()
+                    s.close()
+                    used -= need
+                    del ctx[r.rid]
+                    self.done.append(r)
+                else:
+                    still.append(r)
+            self.active = still
+        wall = now()
+        lat = np.array([r.finished - r.arrival for r in self.done])
+        return {
+            "requests": len(self.done),
+            "wall_s": wall,
+            "p50_s": float(np.percentile(lat, 50)) if len(lat) else None,
+            "p99_s": float(np.percentile(lat, 99)) if len(lat) else None,
+            "throughput_req_per_s": len(self.done) / wall if wall > 0 else None,
+            "gdof_stage_per_s": total_dof_stages / wall / 1e9 if wall > 0 else None,
+        }
+
+
+def poisson_mix(n: int, rate: float, seed: int = 1, frac_2d: float = 0.7, steady_tol: float = 1e-6,
+                unsteady_steps: int = 100, n3d=(8, 16), p3d=(2, 4), nx2d=(16, 48), ny2d=(8, 16),
+                p2d=(1, 3), mixed_frac: float = 0.5) -> list[Request]:
+    """Poisson arrivals at `rate` per second of a 2-D steady / 3-D unsteady mix."""
+    rng = np.random.default_rng(seed)
+    t = 0.0
+    out = []
+    for i in range(n):
+        t += rng.exponential(1.0 / rate)
+        prec = "mixed" if rng.random() < mixed_frac else "fp64"
+        if rng.random() < frac_2d:
+            out.append(Request(i, "bump2d", (int(rng.integers(nx2d[0], nx2d[1] + 1)),
+                                             int(rng.integers(ny2d[0], ny2d[1] + 1))),
+                               int(rng.integers(p2d[0], p2d[1] + 1)), t, mach=float(rng.uniform(0.2, 0.5)),
+                               precision=prec, steady_tol=steady_tol))
+        else:
+            out.append(Request(i, "vortex3d", (int(rng.integers(n3d[0], n3d[1] + 1)),),
+                               int(rng.integers(p3d[0], p3d[1] + 1)), t, mach=0.5, precision=prec,
+                               n_steps=unsteady_steps))
+    return out

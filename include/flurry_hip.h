@@ -110,6 +110,18 @@ typedef struct fhip_desc {
   const int* face_loc_r;
   const int* face_rel_rot;
   const int* face_bc;
+  /* Batching (optional, fused paths only; all NULL/0 for one mesh).  Several
+   * independent meshes of the same order and options may share a context:
+   * their elements are numbered consecutively, mesh m owning elements
+   * [mesh_ele_start[m], mesh_ele_start[m+1]).  Boundary faces may take their
+   * free-stream state from a table instead of params: face_bnd_state[i] is
+   * -1 or a row of bnd_states [n_bnd_states][5] = rho, u, v, w, p.
+   * dt_type 1 then takes the CFL minimum per mesh. */
+  int n_meshes;
+  const int* mesh_ele_start;        /* [n_meshes+1] */
+  int n_bnd_states;
+  const double* bnd_states;         /* [n_bnd_states][5] */
+  const int* face_bnd_state;        /* [n_faces] */
 } fhip_desc;
 
 /* Array ids for fhip_get_array / fhip_set_array / fhip_device_ptr. */
@@ -176,6 +188,17 @@ int fhip_set_option(fhip_ctx* ctx, int option, int value);
 
 /* Residual norm over divF_spts[0] (resType 1, 2 or 3): out[nFields]. */
 int fhip_residual_norm(fhip_ctx* ctx, int res_type, double* out);
+
+/* Batched contexts: residual norm of every mesh, out[n_meshes][nFields],
+ * from divF_spts[0] of the last fused step; and per-mesh activity flags
+ * (0 freezes a mesh: its solution is left unchanged by fhip_update). */
+int fhip_mesh_residual_norms(fhip_ctx* ctx, int res_type, double* out);
+int fhip_set_mesh_active(fhip_ctx* ctx, const int* active);
+
+/* n_steps of fhip_update on several contexts, interleaved step by step so
+ * that their kernels overlap on the device (each context has its own
+ * stream).  Returns after queuing; fhip_sync each context to wait. */
+int fhip_update_many(fhip_ctx** ctxs, int n_ctx, int n_steps, int path);
 
 /* Scan U_spts for NaN/Inf, rho <= 0 or p <= 0; returns FHIP_ESTATE if found. */
 int fhip_check_state(fhip_ctx* ctx);
