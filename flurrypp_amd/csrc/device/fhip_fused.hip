@@ -244,9 +244,11 @@ __global__ __launch_bounds__(256) void k_volume(int E, const double* __restrict_
                                                  Tabs tb, Phys ph) {
   using Sh = Shape<D, P>;
   constexpr int S = Sh::S, F = Sh::F, N = Sh::N, EB = Sh::EB, P1 = Sh::P1, NC = Sh::NC;
+  // sF holds the physical gradient (viscous runs) until the gradient has been
+  // extrapolated to the faces, then the transformed flux at the same indices.
   __shared__ R sU[EB * N * S];
   __shared__ R sF[EB * D * N * S];
-  __shared__ R sG[VISC ? EB * D * N * S : 1];
+  R* const sG = sF;
   const int e0 = blockIdx.x * EB, nb = min(EB, E - e0);
   {
     const double* src = Ue + (long long)e0 * N * S;
@@ -256,26 +258,18 @@ __global__ __launch_bounds__(256) void k_volume(int E, const double* __restrict_
   const int b = threadIdx.x / S, s = threadIdx.x - b * S;
   const bool act = b < nb;
   const long long e = e0 + b;
-  if (act) {
-    R u[N];
-#pragma unroll
-    for (int k = 0; k < N; k++) u[k] = sU[(b * N + k) * S + s];
-    R G[D][D];
-#pragma unroll
-    for (int r = 0; r < D; r++)
-#pragma unroll
-      for (int x = 0; x < D; x++) G[r][x] = R(JGe[((e * D + r) * D + x) * S + s]);
-    R Fl[D][N];
-    inviscid_flux<D, R>(u, Fl, R(ph.gamma));
-    if constexpr (VISC) {
-      // reference-space gradient (opp_grad_spts) plus correction (opp_correctU)
+  if constexpr (VISC) {
+    // phase 1: reference-space gradient (opp_grad_spts) plus correction
+    // (opp_correctU), then the physical gradient (correctGradU transform,
+    // src/solver.cpp:1105-1136), kept in LDS for phase 2 and the faces
+    if (act) {
       R gr[D][N];
 #pragma unroll
       for (int d = 0; d < D; d++) {
         const int* gi = tb.grd_idx + (d * S + s) * P1;
         const double* gw = tb.grd_w + (d * S + s) * P1;
         const double* cw = tb.cu_w + (d * S + s) * NC;
-#pragma unroll
+#pragma unroll 1
         for (int k = 0; k < N; k++) {
           R a = 0;
 #pragma unroll
@@ -286,29 +280,67 @@ __global__ __launch_bounds__(256) void k_volume(int E, const double* __restrict_
           gr[d][k] = c + a;
         }
       }
-      // physical gradient (correctGradU transform, src/solver.cpp:1105-1136)
+      R G[D][D];
+#pragma unroll
+      for (int r = 0; r < D; r++)
+#pragma unroll
+        for (int x = 0; x < D; x++) G[r][x] = R(JGe[((e * D + r) * D + x) * S + s]);
       const R inv = R(1. / detJe[e * S + s]);
-      R g[D][N];
 #pragma unroll
       for (int k = 0; k < N; k++)
 #pragma unroll
         for (int x = 0; x < D; x++) {
           R acc = gr[0][k] * G[0][x] + gr[1][k] * G[1][x];
           if constexpr (D == 3) acc += gr[2][k] * G[2][x];
-          g[x][k] = inv * acc;
+          sG[((b * D + x) * N + k) * S + s] = inv * acc;
         }
+    }
+    __syncthreads();
+    // physical gradient at the flux points (extrapolateGradU)
+    for (int i = threadIdx.x; i < nb * F; i += blockDim.x) {
+      const int bb = i / F, f = i - bb * F;
+      const long long ee = e0 + bb;
+      const int* xi = tb.ext_idx + f * P1;
+      const double* xw = tb.ext_w + f * P1;
+#pragma unroll 1
+      for (int x = 0; x < D; x++)
+#pragma unroll
+        for (int k = 0; k < N; k++) {
+          R a = 0;
+#pragma unroll
+          for (int m = 0; m < P1; m++) a += R(xw[m]) * sG[((bb * D + x) * N + k) * S + xi[m]];
+          dUfe[((ee * D + x) * N + k) * F + f] = a;
+        }
+    }
+    __syncthreads();
+  }
+  if (act) {
+    // phase 2: flux at the point, transformed to the reference element
+    R u[N];
+#pragma unroll
+    for (int k = 0; k < N; k++) u[k] = sU[(b * N + k) * S + s];
+    R Fl[D][N];
+    if constexpr (VISC) {
+      R g[D][N];
 #pragma unroll
       for (int x = 0; x < D; x++)
 #pragma unroll
-        for (int k = 0; k < N; k++) sG[((b * D + x) * N + k) * S + s] = g[x][k];
-      R Fv[D][N];
-      viscous_flux<D, R>(u, g, Fv, ph);
+        for (int k = 0; k < N; k++) g[x][k] = sG[((b * D + x) * N + k) * S + s];
+      R Fi[D][N];
+      viscous_flux<D, R>(u, g, Fl, ph);
+      inviscid_flux<D, R>(u, Fi, R(ph.gamma));
 #pragma unroll
       for (int d = 0; d < D; d++)
 #pragma unroll
-        for (int k = 0; k < N; k++) Fl[d][k] = Fv[d][k] + Fl[d][k];
+        for (int k = 0; k < N; k++) Fl[d][k] = Fl[d][k] + Fi[d][k];
+    } else {
+      inviscid_flux<D, R>(u, Fl, R(ph.gamma));
     }
-    // transformed flux
+    R G[D][D];
+#pragma unroll
+    for (int r = 0; r < D; r++)
+#pragma unroll
+      for (int x = 0; x < D; x++) G[r][x] = R(JGe[((e * D + r) * D + x) * S + s]);
 #pragma unroll
     for (int d1 = 0; d1 < D; d1++)
 #pragma unroll
@@ -354,18 +386,6 @@ __global__ __launch_bounds__(256) void k_volume(int E, const double* __restrict_
         tot = (d == 0) ? a : a + tot;
       }
       disFne[(ee * N + k) * F + f] = tot;
-    }
-    if constexpr (VISC) {
-      const double* xw = tb.ext_w + f * P1;
-#pragma unroll
-      for (int x = 0; x < D; x++)
-#pragma unroll
-        for (int k = 0; k < N; k++) {
-          R a = 0;
-#pragma unroll
-          for (int m = 0; m < P1; m++) a += R(xw[m]) * sG[((bb * D + x) * N + k) * S + xi[m]];
-          dUfe[((ee * D + x) * N + k) * F + f] = a;
-        }
     }
   }
 }
