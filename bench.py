@@ -204,6 +204,11 @@ def main() -> int:
         "forms": forms,
     }
     print(json.dumps(out), flush=True)
+    # one plain copy of the FP64 state (E*S*N doubles read and written) after
+    # the timed region: the known-size kernel that scales rocprofv3's
+    # FETCH_SIZE / WRITE_SIZE counters to bytes
+    s.copyUspts_U0()
+    s.sync()
     s.close()
     return 0
 

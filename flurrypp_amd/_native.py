@@ -109,6 +109,11 @@ def hip_lib() -> C.CDLL:
         lib.fhip_mesh_residual_norms.argtypes = [C.c_void_p, C.c_int, _dp]
         lib.fhip_set_mesh_active.argtypes = [C.c_void_p, _ip]
         lib.fhip_update_many.argtypes = [C.POINTER(C.c_void_p), C.c_int, C.c_int, C.c_int]
+        lib.fhip_pmg_create.argtypes = [C.POINTER(C.c_void_p), C.c_int, C.POINTER(_dp), C.POINTER(_dp), C.c_int,
+                                        C.POINTER(C.c_void_p)]
+        lib.fhip_pmg_cycle.argtypes = [C.c_void_p, C.c_int]
+        lib.fhip_pmg_destroy.argtypes = [C.c_void_p]
+        lib.fhip_pmg_destroy.restype = None
         lib.fhip_array_size.argtypes = [C.c_void_p, C.c_int]
         lib.fhip_array_size.restype = C.c_longlong
         lib.fhip_get_array.argtypes = [C.c_void_p, C.c_int, C.c_int, C.c_void_p]

@@ -161,12 +161,13 @@ def _u01(x: int) -> float:
 
 
 def bump2d(nx: int = 32, ny: int = 12, order: int = 2, mach: float = 0.3, cfl: float = 0.5, dt_type: int = 2,
-           time_type: int = 4, jitter: float = 0.1, seed: int = SEED, iters: int = 100):
+           time_type: int = 4, jitter: float = 0.1, seed: int = SEED, iters: int = 100, dt: float = 1e-3):
     """Subsonic Euler flow over a Gaussian bump: rho=1, p=1/gamma (c=1),
-    inlet velocity (M, 0); local-CFL time stepping by default."""
+    inlet velocity (M, 0); local-CFL time stepping by default (dt is used
+    with dt_type 0 only)."""
     g = 1.4
     p = _params(equation=1, nDims=2, order=order, meshType=0, meshFileName="-", timeType=time_type, dtType=dt_type,
-                CFL=cfl, maxTime=1e9, iterMax=iters, icType=0, riemannType=0, gamma=g, rhoBound=1.0, uBound=mach,
+                CFL=cfl, dt=dt, maxTime=1e9, iterMax=iters, icType=0, riemannType=0, gamma=g, rhoBound=1.0, uBound=mach,
                 vBound=0.0, pBound=1.0 / g, mesh_bound={"fluid": "fluid"})
     m = bump_mesh(nx, ny, jitter, seed)
     setup = HostSetup(p, m)

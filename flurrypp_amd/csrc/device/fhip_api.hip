@@ -417,16 +417,17 @@ static bool resolve_path(fhip::Ctx& c, int path, bool& mixed) {
   return fused;
 }
 
-// One iteration of the main loop body: iter++, solver::update.
-static void update_step(fhip::Ctx& c, bool fused, bool mixed) {
+// One iteration of the main loop body: iter++, solver::update(src).  The
+// coarse P-multigrid levels run update(true) without advancing their counter.
+static void update_step(fhip::Ctx& c, bool fused, bool mixed, bool src = false, bool count = true) {
   const int nRK = c.z.nRK;
-  c.iter++;
-  bool refresh = (c.iter == c.prm.init_iter + 1);
+  if (count) c.iter++;
+  const bool refresh = count && (c.iter == c.prm.init_iter + 1);
   if (fused) {
     // Steady-state steps replay one captured graph; the wave-speed refresh
     // step and timed runs launch kernel by kernel.
     if (!refresh && !c.timing && c.use_graphs) {
-      hipGraphExec_t& ge = c.graph[0][mixed ? 1 : 0];
+      hipGraphExec_t& ge = c.graph[src ? 1 : 0][mixed ? 1 : 0];
       if (!ge) {
         hipGraph_t g = nullptr;
         HCHK(hipStreamBeginCapture(c.stream, hipStreamCaptureModeThreadLocal));
@@ -446,15 +447,15 @@ static void update_step(fhip::Ctx& c, bool fused, bool mixed) {
     if (step == 0 && c.prm.dt_type != 0) fhip::calc_dt(c, refresh);
     if (step == 0) fhip::copy_array(c, c.U0, c.U, c.z.nS());
     fhip::staged_residual(c, step);
-    fhip::staged_time_step_a(c, step, c.prm.rk_a[step + 1], false);
+    fhip::staged_time_step_a(c, step, c.prm.rk_a[step + 1], src);
   }
   fhip::staged_residual(c, nRK - 1);
   if (c.prm.time_type < 5) {
     if (nRK > 1) fhip::copy_array(c, c.U, c.U0, c.z.nS());
     else if (c.prm.dt_type != 0) fhip::calc_dt(c, refresh);
-    for (int step = 0; step < nRK; step++) fhip::staged_time_step_b(c, step, false);
+    for (int step = 0; step < nRK; step++) fhip::staged_time_step_b(c, step, src);
   } else {
-    fhip::staged_time_step_a(c, nRK - 1, c.prm.rk_a[nRK], false);
+    fhip::staged_time_step_a(c, nRK - 1, c.prm.rk_a[nRK], src);
   }
 }
 
@@ -489,6 +490,141 @@ int fhip_update_many(fhip_ctx** hs, int n, int n_steps, int path) {
       for (int i = 0; i < n; i++) update_step(hs[i]->c, fused[i], mixed[i]);
     for (int i = 0; i < n; i++)
       if (fused[i]) fhip::fused_end(hs[i]->c);
+    return 0;
+  });
+}
+
+// ---- P-multigrid -------------------------------------------------------------
+
+struct fhip_pmg {
+  std::vector<fhip_ctx*> lv;
+  std::vector<double*> R, Pro, sol, corr;  // device; R[i]: i+1 -> i, Pro[i]: i -> i+1
+  int smooth = 1;
+  hipEvent_t ev = nullptr;
+};
+
+static void pmg_free(fhip_pmg* mg) {
+  if (!mg) return;
+  if (!mg->lv.empty()) (void)hipSetDevice(mg->lv.back()->c.device);
+  for (auto* v : {&mg->R, &mg->Pro, &mg->sol, &mg->corr})
+    for (double* p : *v)
+      if (p) (void)hipFree(p);
+  if (mg->ev) (void)hipEventDestroy(mg->ev);
+  delete mg;
+}
+
+int fhip_pmg_create(fhip_ctx* const* levels, int n_levels, const double* const* opp_restrict,
+                    const double* const* opp_prolong, int smooth_steps, fhip_pmg** out) {
+  fhip_pmg* mg = nullptr;
+  int rc = guard([&] {
+    require(levels && opp_restrict && opp_prolong && out, "null argument");
+    require(n_levels >= 2, "P-multigrid needs at least two levels");
+    require(smooth_steps >= 0, "smoothSteps must be >= 0");
+    const fhip::Ctx& f = levels[n_levels - 1]->c;
+    for (int i = 0; i < n_levels; i++) {
+      require(levels[i] != nullptr, "null level");
+      const fhip::Ctx& c = levels[i]->c;
+      require(c.z.E == f.z.E && c.z.N == f.z.N && c.z.D == f.z.D && c.device == f.device,
+              "P-multigrid levels must share the mesh, the equation and the device");
+      if (i + 1 < n_levels) {
+        require(c.z.P + 1 == levels[i + 1]->c.z.P, "P-multigrid level orders must ascend by one");
+        require(opp_restrict[i] && opp_prolong[i], "null transfer operator");
+      }
+    }
+    HCHK(hipSetDevice(f.device));
+    mg = new fhip_pmg();
+    mg->lv.assign(levels, levels + n_levels);
+    mg->smooth = smooth_steps;
+    for (int i = 0; i + 1 < n_levels; i++) {
+      fhip::Ctx& c = levels[i]->c;
+      const size_t Sc = c.z.S, Sf = levels[i + 1]->c.z.S, nS = c.z.nS();
+      mg->R.push_back(dalloc<double>(Sc * Sf));
+      HCHK(hipMemcpy(mg->R.back(), opp_restrict[i], Sc * Sf * sizeof(double), hipMemcpyHostToDevice));
+      mg->Pro.push_back(dalloc<double>(Sc * Sf));
+      HCHK(hipMemcpy(mg->Pro.back(), opp_prolong[i], Sc * Sf * sizeof(double), hipMemcpyHostToDevice));
+      mg->sol.push_back(dalloc<double>(nS));
+      mg->corr.push_back(dalloc<double>(nS));
+      if (!c.src) c.src = dalloc<double>(nS);
+    }
+    HCHK(hipEventCreateWithFlags(&mg->ev, hipEventDisableTiming));
+    *out = mg;
+    return 0;
+  });
+  if (rc != 0) {
+    pmg_free(mg);
+    if (out) *out = nullptr;
+  }
+  return rc;
+}
+
+void fhip_pmg_destroy(fhip_pmg* mg) { pmg_free(mg); }
+
+// Orders work across two levels' streams.
+static void pmg_handoff(fhip_pmg* mg, fhip::Ctx& from, fhip::Ctx& to) {
+  HCHK(hipEventRecord(mg->ev, from.stream));
+  HCHK(hipStreamWaitEvent(to.stream, mg->ev, 0));
+}
+
+static void pmg_residual(fhip::Ctx& c, bool fused, bool mixed) {
+  if (fused) fhip::fused_residual(c, mixed);
+  else fhip::staged_residual(c, 0);
+}
+
+static void pmg_smooth(fhip::Ctx& c, int n, bool fused, bool mixed) {
+  if (n <= 0) return;
+  if (fused) fhip::fused_begin(c, mixed, true);
+  for (int i = 0; i < n; i++) update_step(c, fused, mixed, true, false);
+  if (fused) fhip::fused_end(c);
+}
+
+// multiGrid::restrict_pmg (src/multigrid.cpp:315-345), on the coarse stream
+static void pmg_restrict(fhip_pmg* mg, int i) {
+  fhip::Ctx& f = mg->lv[i + 1]->c;
+  fhip::Ctx& c = mg->lv[i]->c;
+  pmg_handoff(mg, f, c);
+  fhip::op_apply(c, "pmg_restrict", c.z.S, f.z.S, mg->R[i], f.U, c.U, false);
+  fhip::op_apply(c, "pmg_restrict", c.z.S, f.z.S, mg->R[i], f.divF, c.divF, false);
+}
+
+// multiGrid::cycle (src/multigrid.cpp:169-313), P levels.
+int fhip_pmg_cycle(fhip_pmg* mg, int path) {
+  return guard([&] {
+    require(mg, "null multigrid");
+    const int n = (int)mg->lv.size();
+    std::vector<char> fused(n), mixed(n);
+    for (int i = 0; i < n; i++) {
+      bool m;
+      fused[i] = resolve_path(mg->lv[i]->c, path, m);
+      mixed[i] = m;
+    }
+    fhip::Ctx& fine = mg->lv[n - 1]->c;
+    HCHK(hipSetDevice(fine.device));
+    pmg_residual(fine, fused[n - 1], mixed[n - 1]);
+    pmg_restrict(mg, n - 2);
+    for (int i = n - 2; i >= 0; i--) {
+      fhip::Ctx& g = mg->lv[i]->c;
+      const long long nS = g.z.nS();
+      // compute_source_term (src/multigrid.cpp:381-398)
+      fhip::copy_array(g, g.src, g.divF, nS);
+      pmg_residual(g, fused[i], mixed[i]);
+      fhip::sub_array(g, g.src, g.src, g.divF, nS);
+      fhip::copy_array(g, mg->sol[i], g.U, nS);
+      pmg_smooth(g, mg->smooth, fused[i], mixed[i]);
+      if (i > 0) {
+        pmg_residual(g, fused[i], mixed[i]);
+        fhip::add_array(g, g.divF, g.src, nS);
+        pmg_restrict(mg, i - 1);
+      }
+    }
+    // upward: smooth again, corr = U - sol, prolong_err (src/multigrid.cpp:347-364)
+    for (int i = 0; i + 1 < n; i++) {
+      fhip::Ctx& g = mg->lv[i]->c;
+      fhip::Ctx& up = mg->lv[i + 1]->c;
+      pmg_smooth(g, mg->smooth, fused[i], mixed[i]);
+      fhip::sub_array(g, mg->corr[i], g.U, mg->sol[i], g.z.nS());
+      pmg_handoff(mg, g, up);
+      fhip::op_apply(up, "pmg_prolong", up.z.S, g.z.S, mg->Pro[i], mg->corr[i], up.U, true);
+    }
     return 0;
   });
 }

@@ -94,13 +94,16 @@ struct FusedState {
   std::vector<int> h_meshStart;
   double *bst = nullptr, *meshDt = nullptr, *meshRes = nullptr;
   int *eleMesh = nullptr, *meshStart = nullptr, *active = nullptr;
+  // P-multigrid source term (element-major copy of src_spts), used while use_src
+  double* srce = nullptr;
+  bool use_src = false;
   Work<double> w64;
   Work<float> w32;
   ~FusedState() {
     w64.release();
     w32.release();
     void* ps[] = {tab_mem, Ue,  U0e,    Uacc,   divF0e,    JGe,    detJe, norme, dAe, waveSpe,
-                  pL,      pR,  bc,     bst,    meshDt,    meshRes, eleMesh, meshStart, active};
+                  pL,      pR,  bc,     bst,    meshDt,    meshRes, eleMesh, meshStart, active, srce};
     for (void* p : ps)
       if (p) (void)hipFree(p);
   }
@@ -461,12 +464,29 @@ __global__ __launch_bounds__(256) void k_rk(int E, int mode, double a, double bc
                                              double* __restrict__ U0e, double* __restrict__ Uacc,
                                              double* __restrict__ divF0e, R* __restrict__ Ufe, Tabs tb,
                                              const int* __restrict__ eleMesh, const double* __restrict__ meshDt,
-                                             const int* __restrict__ active) {
+                                             const int* __restrict__ active, const double* __restrict__ srce) {
   using Sh = Shape<D, P>;
   constexpr int S = Sh::S, F = Sh::F, N = Sh::N, EB = Sh::EB, P1 = Sh::P1, NC = Sh::NC;
   __shared__ R sU[EB * N * S];
   const int e0 = blockIdx.x * EB, nb = min(EB, E - e0);
   const int b = threadIdx.x / S, s = threadIdx.x - b * S;
+  if (mode == 5) {
+    // residual only (calcResidual(0) + correctDivFlux): divF_spts[0], no update
+    if (b < nb) {
+      const long long e = e0 + b;
+#pragma unroll
+      for (int k = 0; k < N; k++) {
+        R c = 0;
+#pragma unroll
+        for (int j = 0; j < NC; j++) {
+          const long long jf = (e * N + k) * F + tb.cor_idx[s * NC + j];
+          c += R(tb.cor_w[s * NC + j]) * (Fne[jf] - disFne[jf]);
+        }
+        divF0e[(e * N + k) * S + s] = double(c + divFe[(e * N + k) * S + s]);
+      }
+    }
+    return;  // uniform over the block: no barrier follows for anyone
+  }
   if (b < nb) {
     const long long e = e0 + b;
     // dt: 0 global, 1 per element (local CFL), 2 per mesh (batched global CFL)
@@ -482,8 +502,10 @@ __global__ __launch_bounds__(256) void k_rk(int E, int mode, double a, double bc
         const long long jf = (e * N + k) * F + tb.cor_idx[s * NC + j];
         c += R(tb.cor_w[s * NC + j]) * (Fne[jf] - disFne[jf]);
       }
-      const double r = double(c + divFe[iu]);
-      if (divF0e) divF0e[iu] = r;
+      const double r0 = double(c + divFe[iu]);
+      if (divF0e) divF0e[iu] = r0;
+      // P-multigrid source: divF_spts[step] + src_spts (src/solver.cpp:480,514)
+      const double r = srce ? r0 + srce[iu] : r0;
       const double u = Ue[iu];
       double un;
       if (mode == 0) {
@@ -935,7 +957,8 @@ struct Runner {
     hipLaunchKernelGGL((k_rk<D, P, R>), dim3(nblk(c.z.E, Sh::EB)), dim3(Sh::NT), 0, c.stream, c.z.E, mode, a, b, w.divFe,
                        w.Fne, w.disFne, st.detJe, c.d_dt, c.eleDt,
                        c.prm.dt_type == 2 ? 1 : (c.prm.dt_type == 1 && st.nMesh > 1 ? 2 : 0), st.Ue, st.U0e, st.Uacc,
-                       write0 ? st.divF0e : nullptr, w.Ufe, st.tb, st.eleMesh, st.meshDt, st.active);
+                       write0 ? st.divF0e : nullptr, w.Ufe, st.tb, st.eleMesh, st.meshDt, st.active,
+                       st.use_src ? st.srce : nullptr);
     FCHK(hipGetLastError());
   }
 };
@@ -984,13 +1007,19 @@ static double cfl_limit_f(int order) {  // getCFLLimit, src/global.cpp:81-105
   return t[order];
 }
 
-void fused_begin(Ctx& c, bool mixed) {
+void fused_begin(Ctx& c, bool mixed, bool src) {
   FusedState& st = fs(c);
   const Sizes& z = c.z;
   Ops o = get_ops(c, mixed);
   o.ensure(c);
   long long nS = (long long)z.E * z.N * z.S;
   hipLaunchKernelGGL(k_to_em, dim3(nblk(nS, 256)), dim3(256), 0, c.stream, nS, z.S, z.E, z.N, c.U, st.Ue);
+  st.use_src = src;
+  if (src) {
+    if (!c.src) throw std::invalid_argument("no P-multigrid source term on this context");
+    if (!st.srce) FCHK(hipMalloc(&st.srce, nS * sizeof(double)));
+    hipLaunchKernelGGL(k_to_em, dim3(nblk(nS, 256)), dim3(256), 0, c.stream, nS, z.S, z.E, z.N, c.src, st.srce);
+  }
   long long nW = (long long)z.E * z.F;
   hipLaunchKernelGGL(k_to_em, dim3(nblk(nW, 256)), dim3(256), 0, c.stream, nW, z.F, z.E, 1, c.waveSp, st.waveSpe);
   FCHK(hipGetLastError());
@@ -999,9 +1028,26 @@ void fused_begin(Ctx& c, bool mixed) {
 
 void fused_end(Ctx& c) {
   FusedState& st = fs(c);
+  st.use_src = false;
   const Sizes& z = c.z;
   long long nS = (long long)z.E * z.N * z.S;
   hipLaunchKernelGGL(k_from_em, dim3(nblk(nS, 256)), dim3(256), 0, c.stream, nS, z.S, z.E, z.N, st.Ue, c.U);
+  hipLaunchKernelGGL(k_from_em, dim3(nblk(nS, 256)), dim3(256), 0, c.stream, nS, z.S, z.E, z.N, st.divF0e, c.divF);
+  long long nW = (long long)z.E * z.F;
+  hipLaunchKernelGGL(k_from_em, dim3(nblk(nW, 256)), dim3(256), 0, c.stream, nW, z.F, z.E, 1, st.waveSpe, c.waveSp);
+  FCHK(hipGetLastError());
+}
+
+// calcResidual(0) at the current state (src/solver.cpp:366-445) on a fused
+// path: divF_spts[0] and the face wave speeds in FlurryPP's layouts.
+void fused_residual(Ctx& c, bool mixed) {
+  FusedState& st = fs(c);
+  const Sizes& z = c.z;
+  Ops o = get_ops(c, mixed);
+  fused_begin(c, mixed, false);
+  o.residual(c);
+  o.rk(c, 5, 0., 0., true);
+  long long nS = (long long)z.E * z.N * z.S;
   hipLaunchKernelGGL(k_from_em, dim3(nblk(nS, 256)), dim3(256), 0, c.stream, nS, z.S, z.E, z.N, st.divF0e, c.divF);
   long long nW = (long long)z.E * z.F;
   hipLaunchKernelGGL(k_from_em, dim3(nblk(nW, 256)), dim3(256), 0, c.stream, nW, z.F, z.E, 1, st.waveSpe, c.waveSp);

@@ -109,15 +109,22 @@ void staged_time_step_a(Ctx& c, int step, double rk, bool src);
 void staged_time_step_b(Ctx& c, int step, bool src);
 void copy_array(Ctx& c, double* dst, const double* src, long long n);
 void calc_dt(Ctx& c, bool refresh);
+// C[m][E*N] = A[m][k] B[k][E*N] (+ C when beta): an operator applied to every
+// element and field of a FlurryPP-layout array
+void op_apply(Ctx& c, const char* name, int m, int k, const double* A, const double* B, double* C, bool beta);
+void sub_array(Ctx& c, double* out, const double* a, const double* b, long long n);  // out = a; out -= b
+void add_array(Ctx& c, double* y, const double* x, long long n);                     // y += x
 void residual_norm(Ctx& c, int res_type, double* out_host);
 int check_state(Ctx& c);
 
 // fused path (fhip_fused.hip)
 bool fused_setup(Ctx& c, const fhip_desc* d);  // false: configuration not covered
 void fused_release(Ctx& c);
-void fused_begin(Ctx& c, bool mixed);  // import U and wave speeds, extrapolate U to the faces
+// import U, wave speeds [and the P-multigrid source], extrapolate U to the faces
+void fused_begin(Ctx& c, bool mixed, bool src = false);
 void fused_end(Ctx& c);                // export U, divF_spts[0] and wave speeds
 void fused_rk_step(Ctx& c, bool refresh, bool mixed);
+void fused_residual(Ctx& c, bool mixed);  // divF_spts[0] and wave speeds of the current U
 void fused_mesh_residuals(Ctx& c, int type, double* out);  // out [nMesh][N]
 int fused_n_meshes(Ctx& c);
 void fused_set_active(Ctx& c, const int* active);

@@ -392,6 +392,37 @@ __global__ void k_check(long long nPts, const double* __restrict__ U, double gam
 
 // ---------------------------------------------------------------------------
 
+void op_apply(Ctx& c, const char* name, int m, int k, const double* A, const double* B, double* C, bool beta) {
+  opgemm(c, name, m, k, A, B, C, beta);
+}
+
+// out may alias a
+__global__ void k_sub_arr(long long n, double* out, const double* a, const double* __restrict__ b) {
+  long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n) {
+    double v = a[i];
+    v -= b[i];
+    out[i] = v;
+  }
+}
+
+__global__ void k_add_arr(long long n, double* __restrict__ y, const double* __restrict__ x) {
+  long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n) y[i] += x[i];
+}
+
+void sub_array(Ctx& c, double* out, const double* a, const double* b, long long n) {
+  Launch L(c, "pmg_sub");
+  hipLaunchKernelGGL(k_sub_arr, dim3(blocks_for(n, 256)), dim3(256), 0, c.stream, n, out, a, b);
+  HIPCHK(hipGetLastError());
+}
+
+void add_array(Ctx& c, double* y, const double* x, long long n) {
+  Launch L(c, "pmg_add");
+  hipLaunchKernelGGL(k_add_arr, dim3(blocks_for(n, 256)), dim3(256), 0, c.stream, n, y, x);
+  HIPCHK(hipGetLastError());
+}
+
 void copy_array(Ctx& c, double* dst, const double* src, long long n) {
   Launch L(c, "copy");
   hipLaunchKernelGGL(k_copy, dim3(blocks_for(n, 256)), dim3(256), 0, c.stream, n, dst, src);

@@ -356,4 +356,33 @@ void operators(int nDims, int order, int ptsType, int vcjh, Operators& op) {
   op.w_face = weightsNd(order, nDims - 1);
 }
 
+// P-multigrid transfer operators of one order (oper::setupPMG,
+// src/operators.cpp:1553-1634): the degree-`order` interpolant through the
+// solution points, evaluated at the solution points of order+1 (prolongation,
+// [(order+2)^D][S]) and of order-1 (restriction, [order^D][S]; empty at order 0).
+// Products run x, y, z as in the reference; point p of a tensor grid with n
+// points per direction has indices (p % n, p / n % n, p / n^2).
+void pmg_operators(int nDims, int order, int ptsType, std::vector<double>& prolong, std::vector<double>& restrict_) {
+  const int P1 = order + 1;
+  const int S = nDims == 2 ? P1 * P1 : P1 * P1 * P1;
+  const std::vector<double> s1 = pts1d(ptsType, order);
+  auto build = [&](int n1, std::vector<double>& op) {
+    const std::vector<double> t1 = pts1d(ptsType, n1 - 1);
+    const int T = nDims == 2 ? n1 * n1 : n1 * n1 * n1;
+    op.assign((size_t)T * S, 0.);
+    for (int spt = 0; spt < S; spt++) {
+      const int is = spt % P1, js = (spt / P1) % P1, ks = spt / (P1 * P1);
+      for (int t = 0; t < T; t++) {
+        const int it = t % n1, jt = (t / n1) % n1, kt = t / (n1 * n1);
+        double v = lagrange(s1, t1[it], is) * lagrange(s1, t1[jt], js);
+        if (nDims == 3) v = v * lagrange(s1, t1[kt], ks);
+        op[(size_t)t * S + spt] = v;
+      }
+    }
+  };
+  build(order + 2, prolong);
+  if (order > 0) build(order, restrict_);
+  else restrict_.clear();
+}
+
 }  // namespace fr

@@ -41,5 +41,6 @@ void sizes(int nDims, int order, int& nSpts, int& nFpts, int& nFptsFace);
 void face_tnorm(int nDims, int iFace, double* t);
 void ref_points(int nDims, int order, int ptsType, std::vector<Pt>& spts, std::vector<Pt>& fpts);
 void operators(int nDims, int order, int ptsType, int vcjh, Operators& op);
+void pmg_operators(int nDims, int order, int ptsType, std::vector<double>& prolong, std::vector<double>& restrict_);
 
 }  // namespace fr

@@ -543,6 +543,17 @@ int fhost_operators(int nDims, int order, int ptsType, int vcjh, double* spts_to
   });
 }
 
+int fhost_pmg_operators(int nDims, int order, int ptsType, double* prolong, double* restrict_op) {
+  return guard([&] {
+    if (nDims != 2 && nDims != 3) throw Error("nDims must be 2 or 3");
+    if (order < 0 || order > 9) throw Error("P-multigrid operators need 0 <= order <= 9");
+    std::vector<double> pro, res;
+    fr::pmg_operators(nDims, order, ptsType, pro, res);
+    if (prolong) std::copy(pro.begin(), pro.end(), prolong);
+    if (restrict_op) std::copy(res.begin(), res.end(), restrict_op);
+  });
+}
+
 int fhost_mesh_cartesian(int nDims, int nx, int ny, int nz, const double* lo, const double* hi,
                          const char* const* bc_names, fhost_mesh** out) {
   return guard([&] {

@@ -223,6 +223,9 @@ class InputParams:
         p.scFlag = g("shockCapture", int, 0)
         p.squeeze = g("squeeze", int, 0)
         p.PMG = g("PMG", int, 0)
+        if p.PMG:
+            p.lowOrder = g("lowOrder", int, 0)
+            p.smoothSteps = g("smoothSteps", int, 1)
         p.HMG = g("HMG", int, 0)
         p.initIter = p.restartIter if p.restart else 0
         p.iter = p.initIter

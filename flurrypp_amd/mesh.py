@@ -52,6 +52,18 @@ def operators(nDims: int, order: int, spts_type: str = "Legendre", vcjh: int = 0
     return Operators(nDims, order, **a)
 
 
+def pmg_operators(nDims: int, order: int, spts_type: str = "Legendre"):
+    """P-multigrid transfer operators of one order (oper::setupPMG,
+    src/operators.cpp:1553-1634): prolongation [(P+2)^D][S] to order P+1 and
+    restriction [P^D][S] to order P-1 (None at order 0)."""
+    S = (order + 1) ** nDims
+    pro = np.zeros(((order + 2) ** nDims, S))
+    res = np.zeros((order ** nDims, S)) if order > 0 else None
+    host_check(host_lib().fhost_pmg_operators(nDims, order, PTS[spts_type.lower()], dptr(pro),
+                                              dptr(res) if res is not None else None))
+    return pro, res
+
+
 def pts1d(order: int, spts_type: str = "Legendre") -> np.ndarray:
     out = np.zeros(order + 1)
     host_check(host_lib().fhost_pts1d(PTS[spts_type.lower()], order, dptr(out)))

@@ -8,6 +8,7 @@ comes from libflurry_host.
 """
 from __future__ import annotations
 
+import copy
 import ctypes as C
 import math
 
@@ -15,7 +16,7 @@ import numpy as np
 
 from . import _native as nat
 from .input import InputParams
-from .mesh import Mesh, bc_number, operators, sizes
+from .mesh import Mesh, bc_number, operators, pmg_operators, sizes
 
 PI = 4.0 * math.atan(1.0)  # include/global.hpp:72
 
@@ -113,7 +114,8 @@ class HostSetup:
         p = params
         self.params = p
         self.mesh = mesh if mesh is not None else build_mesh(p)
-        self.mesh.process()
+        if not getattr(self.mesh, "processed", False):
+            self.mesh.process()
         self.nDims, self.nFields, self.order = p.nDims, p.nFields, p.order
         self.nEles = self.mesh.nEles
         self.nSpts, self.nFpts, self.nFptsFace = sizes(self.nDims, self.order)
@@ -297,13 +299,74 @@ class Solver:
         return dict(opers=o, geo=g, faces=f, params=self._fp)
 
 
+class MultiGrid:
+    """P-multigrid (class multiGrid, src/multigrid.cpp:46-143,169-313): coarse
+    solvers of orders lowOrder..P-1 on the fine solver's mesh, one cycle after
+    every fine update.  As in multiGrid::setup, each coarse level gets a copy of
+    the fine input (its RK coefficients included) with only the order changed,
+    and its own initial condition and wave speeds."""
+
+    def __init__(self, fine: Solver, low_order: int | None = None, smooth_steps: int | None = None):
+        p = fine.params
+        low = getattr(p, "lowOrder", 0) if low_order is None else low_order
+        self.smooth = getattr(p, "smoothSteps", 1) if smooth_steps is None else smooth_steps
+        if not 0 <= low < fine.order:
+            raise ValueError("P-multigrid needs 0 <= lowOrder < order")
+        self.fine = fine
+        self.coarse: list[Solver] = []
+        for P in range(low, fine.order):
+            pc = copy.copy(p)
+            pc.order = P
+            s = Solver(pc, setup=HostSetup(pc, fine.mesh))
+            s.initializeSolution()
+            self.coarse.append(s)
+        levels = self.coarse + [fine]
+        D = fine.nDims
+        sp = p.sptsTypeQuad
+        self.restrict_ops = [pmg_operators(D, lv.order, sp)[1] for lv in levels[1:]]
+        self.prolong_ops = [pmg_operators(D, lv.order, sp)[0] for lv in levels[:-1]]
+        n = len(levels)
+        hs = (C.c_void_p * n)(*[lv._ctx.value for lv in levels])
+        R = (nat._dp * (n - 1))(*[nat.dptr(a) for a in self.restrict_ops])
+        Pr = (nat._dp * (n - 1))(*[nat.dptr(a) for a in self.prolong_ops])
+        h = C.c_void_p()
+        nat.hip_check(nat.hip_lib().fhip_pmg_create(hs, n, R, Pr, self.smooth, C.byref(h)))
+        self._mg = h
+
+    @property
+    def levels(self) -> list[Solver]:
+        return self.coarse + [self.fine]
+
+    def cycle(self, path: int = nat.PATH_AUTO, sync: bool = True) -> None:
+        nat.hip_check(nat.hip_lib().fhip_pmg_cycle(self._mg, path))
+        if sync:
+            self.fine.sync()
+
+    def close(self) -> None:
+        if getattr(self, "_mg", None) is not None and self._mg.value:
+            nat.hip_lib().fhip_pmg_destroy(self._mg)
+            self._mg = None
+        for s in self.coarse:
+            s.close()
+        self.coarse = []
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
 def run(params: InputParams, log=print) -> Solver:
     """The main loop of src/flurry.cpp:120-155 (residual monitor included)."""
     s = Solver(params)
     s.initializeSolution()
     p = params
+    mg = MultiGrid(s) if getattr(p, "PMG", 0) else None
     while s.iter < p.iterMax and p.time < p.maxTime:
         s.update(1)
+        if mg is not None:
+            mg.cycle()
         p.time += s.dt
         it = s.iter
         if it % p.monitorResFreq == 0 or it == p.initIter + 1:

@@ -28,6 +28,8 @@
  *   fhip_residual_norm    ele::getNormResidual summed as in writeResidual
  *                         (src/ele.cpp:1922-1945, src/output.cpp:1035-1089)
  *   fhip_get_array        reads any of the solver arrays named in FHIP_ARR_*
+ *   fhip_pmg_create       multiGrid::setup, P levels      (src/multigrid.cpp:46-143)
+ *   fhip_pmg_cycle        multiGrid::cycle(Solver)         (src/multigrid.cpp:169-313)
  *
  * Layouts are FlurryPP's row-major ones: U_spts [nSpts][nEles][nFields],
  * U_fpts/Fn_fpts/disFn_fpts/dUc_fpts [nFpts][nEles][nFields],
@@ -199,6 +201,25 @@ int fhip_set_mesh_active(fhip_ctx* ctx, const int* active);
  * that their kernels overlap on the device (each context has its own
  * stream).  Returns after queuing; fhip_sync each context to wait. */
 int fhip_update_many(fhip_ctx** ctxs, int n_ctx, int n_steps, int path);
+
+/* ---- P-multigrid (class multiGrid, include/multigrid.hpp:40-64) ----------
+ * levels[0..n_levels-1] are contexts on one mesh whose orders ascend by one;
+ * levels[n_levels-1] is the fine solver, the others are the coarse pGrids
+ * (the caller sets each one up and initialises it, as multiGrid::setup does,
+ * with the fine level's RK coefficients).  opp_restrict[i] maps level i+1 to
+ * level i: row-major [nSpts(i)][nSpts(i+1)] (opers[P].opp_restrict of level
+ * i+1); opp_prolong[i] maps level i to level i+1: [nSpts(i+1)][nSpts(i)]
+ * (opers[P].opp_prolong of level i); both as fhost_pmg_operators builds them.
+ * smooth_steps: input key smoothSteps.  The coarse levels keep their own
+ * src_spts/sol_spts/corr_spts and never refresh their wave speeds in calcDt
+ * (their iteration counter stays at initIter, src/flurry.cpp:131-136).
+ * fhip_pmg_cycle queues one cycle after the fine level's update; every level
+ * must be driven through the same path.  The contexts must outlive mg. */
+typedef struct fhip_pmg fhip_pmg;
+int fhip_pmg_create(fhip_ctx* const* levels, int n_levels, const double* const* opp_restrict,
+                    const double* const* opp_prolong, int smooth_steps, fhip_pmg** out);
+int fhip_pmg_cycle(fhip_pmg* mg, int path);
+void fhip_pmg_destroy(fhip_pmg* mg);
 
 /* Scan U_spts for NaN/Inf, rho <= 0 or p <= 0; returns FHIP_ESTATE if found. */
 int fhip_check_state(fhip_ctx* ctx);

@@ -73,6 +73,14 @@ int fhost_operators(int nDims, int order, int ptsType, int vcjh,
                     double* correction, double* correct_u,
                     double* w_spts, double* w_face);
 
+/* P-multigrid transfer operators of one order (oper::setupPMG,
+ * src/operators.cpp:1553-1634; opp_prolong / opp_restrict,
+ * include/operators.hpp:132-133), row-major:
+ *   prolong     [(order+2)^nDims][nSpts]  order -> order+1
+ *   restrict_op [order^nDims][nSpts]      order -> order-1 (untouched at order 0)
+ * Either pointer may be NULL. */
+int fhost_pmg_operators(int nDims, int order, int ptsType, double* prolong, double* restrict_op);
+
 /* ---- meshes ------------------------------------------------------------- */
 
 /* Cartesian mesh with createMesh's vertex, element and boundary numbering

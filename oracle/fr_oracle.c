@@ -834,12 +834,17 @@ int or_get_iter(or_ctx* c) { return c->iter; }
 
 /* flurry main loop body (src/flurry.cpp:135-138) around solver::update
  * (src/solver.cpp:319-363), static grid */
-int or_update(or_ctx* c, int n_steps, const double* src) {
+/* solver::update (src/solver.cpp:319-363).  `count` advances the iteration
+ * counter that the main loop keeps (src/flurry.cpp:131-136); the P-multigrid
+ * levels below the fine one never advance theirs, so their wave speeds are
+ * never refreshed by calcDt (src/solver.cpp:449-453). */
+static void update_impl(or_ctx* c, int n_steps, const double* src, int count) {
   int nRK = c->nRK;
   for (int it = 0; it < n_steps; it++) {
-    c->iter++;
+    if (count) c->iter++;
+    const int refresh = count && c->iter == c->p.init_iter + 1;
     for (int step = 0; step < nRK - 1; step++) {
-      if (step == 0 && c->p.dt_type != 0) or_calc_dt(c, c->iter == c->p.init_iter + 1, NULL);
+      if (step == 0 && c->p.dt_type != 0) or_calc_dt(c, refresh, NULL);
       if (step == 0) or_copy_u_to_u0(c);
       or_calc_residual(c, step);
       or_time_step_a(c, step, c->p.rk_a[step + 1], src);
@@ -847,12 +852,85 @@ int or_update(or_ctx* c, int n_steps, const double* src) {
     or_calc_residual(c, nRK - 1);
     if (c->p.time_type < 5) {
       if (nRK > 1) or_copy_u0_to_u(c);
-      else if (c->p.dt_type != 0) or_calc_dt(c, c->iter == c->p.init_iter + 1, NULL);
+      else if (c->p.dt_type != 0) or_calc_dt(c, refresh, NULL);
       for (int step = 0; step < nRK; step++) or_time_step_b(c, step, src);
     } else {
       or_time_step_a(c, nRK - 1, c->p.rk_a[nRK], src);
     }
   }
+}
+
+int or_update(or_ctx* c, int n_steps, const double* src) {
+  update_impl(c, n_steps, src, 1);
+  return 0;
+}
+
+/* ---- P-multigrid (src/multigrid.cpp) ----------------------------------- */
+
+/* multiGrid::restrict_pmg, src/multigrid.cpp:315-345: U and divF_spts[0] */
+static void pmg_restrict(or_ctx* f, or_ctx* c, const double* R, double* tmp) {
+  long long n = (long long)c->E * c->N;
+  gemm(c->S, n, f->S, R, f->U, c->U, 0, tmp);
+  gemm(c->S, n, f->S, R, f->divF, c->divF, 0, tmp);
+}
+
+/* multiGrid::cycle, src/multigrid.cpp:169-313, P levels only (the h levels
+ * are 2-D, P=0 only and off by default).  lv[0..nl-1] ascend in order by one;
+ * lv[nl-1] is the fine solver.  compute_source_term (:381-398), prolong_err
+ * (:347-364) and the sol/corr bookkeeping (:182-188,282-292) inline. */
+int or_pmg_cycle(or_ctx** lv, int nl, const double* const* R, const double* const* Pro, int smooth) {
+  if (nl < 2) return fail("P-multigrid needs at least two levels");
+  for (int i = 0; i + 1 < nl; i++)
+    if (lv[i]->E != lv[nl - 1]->E || lv[i]->N != lv[nl - 1]->N || lv[i]->D != lv[nl - 1]->D ||
+        lv[i]->p.order + 1 != lv[i + 1]->p.order)
+      return fail("P-multigrid levels must share the mesh and ascend in order by one");
+  const long long n = (long long)lv[0]->E * lv[0]->N;
+  double* tmp = (double*)malloc(sizeof(double) * n);
+  double** sol = (double**)calloc(nl, sizeof(double*));
+  double** corr = (double**)calloc(nl, sizeof(double*));
+  double** src = (double**)calloc(nl, sizeof(double*));
+  for (int i = 0; i + 1 < nl; i++) {
+    size_t sz = sizeof(double) * (size_t)lv[i]->S * n;
+    sol[i] = (double*)malloc(sz);
+    corr[i] = (double*)malloc(sz);
+    src[i] = (double*)malloc(sz);
+  }
+  or_ctx* fine = lv[nl - 1];
+  or_calc_residual(fine, 0);
+  pmg_restrict(fine, lv[nl - 2], R[nl - 2], tmp);
+  for (int i = nl - 2; i >= 0; i--) {
+    or_ctx* g = lv[i];
+    const long long m = (long long)g->S * n;
+    for (long long j = 0; j < m; j++) src[i][j] = g->divF[j];
+    or_calc_residual(g, 0);
+    for (long long j = 0; j < m; j++) src[i][j] -= g->divF[j];
+    for (long long j = 0; j < m; j++) sol[i][j] = g->U[j];
+    update_impl(g, smooth, src[i], 0);
+    if (i > 0) {
+      or_calc_residual(g, 0);
+      for (long long j = 0; j < m; j++) g->divF[j] += src[i][j];
+      pmg_restrict(g, lv[i - 1], R[i - 1], tmp);
+    }
+  }
+  for (int i = 0; i + 1 < nl; i++) {
+    or_ctx* g = lv[i];
+    const long long m = (long long)g->S * n;
+    update_impl(g, smooth, src[i], 0);
+    for (long long j = 0; j < m; j++) {
+      corr[i][j] = g->U[j];
+      corr[i][j] -= sol[i][j];
+    }
+    gemm(lv[i + 1]->S, n, g->S, Pro[i], corr[i], lv[i + 1]->U, 1, tmp);
+  }
+  for (int i = 0; i + 1 < nl; i++) {
+    free(sol[i]);
+    free(corr[i]);
+    free(src[i]);
+  }
+  free(sol);
+  free(corr);
+  free(src);
+  free(tmp);
   return 0;
 }
 

@@ -65,6 +65,9 @@ int or_copy_u0_to_u(or_ctx* c);
 int or_calc_dt(or_ctx* c, int refresh, double* dt_out);
 int or_set_dt(or_ctx* c, double dt);
 int or_update(or_ctx* c, int n_steps, const double* src);
+/* one P-multigrid cycle over levels lv[0..nl-1] (ascending order, lv[nl-1]
+ * fine); R[i]: level i+1 -> i, Pro[i]: level i -> i+1, row-major. */
+int or_pmg_cycle(or_ctx** lv, int nl, const double* const* R, const double* const* Pro, int smooth);
 int or_residual_norm(or_ctx* c, int res_type, double* out);
 int or_get_iter(or_ctx* c);
 

@@ -72,6 +72,7 @@ def lib() -> C.CDLL:
         L.or_set_dt.argtypes = [C.c_void_p, C.c_double]
         L.or_update.argtypes = [C.c_void_p, C.c_int, _dp]
         L.or_residual_norm.argtypes = [C.c_void_p, C.c_int, _dp]
+        L.or_pmg_cycle.argtypes = [C.POINTER(C.c_void_p), C.c_int, C.POINTER(_dp), C.POINTER(_dp), C.c_int]
         _lib = L
     return _lib
 
@@ -191,3 +192,16 @@ class Oracle:
     @property
     def iter(self):
         return lib().or_get_iter(self._h)
+
+
+def pmg_cycle(levels: list, restrict_ops: list, prolong_ops: list, smooth_steps: int) -> None:
+    """One P-multigrid cycle (multiGrid::cycle, src/multigrid.cpp:169-313)
+    over oracle contexts of ascending order (last = fine).  restrict_ops[i]
+    maps level i+1 to level i, prolong_ops[i] level i to level i+1."""
+    n = len(levels)
+    hs = (C.c_void_p * n)(*[lv._h.value for lv in levels])
+    R = [_d(r) for r in restrict_ops]
+    Pr = [_d(p) for p in prolong_ops]
+    Ra = (_dp * max(1, n - 1))(*[r.ctypes.data_as(_dp) for r in R])
+    Pa = (_dp * max(1, n - 1))(*[p.ctypes.data_as(_dp) for p in Pr])
+    _chk(lib().or_pmg_cycle(hs, n, Ra, Pa, smooth_steps))

@@ -1,0 +1,132 @@
+"""P-multigrid: transfer operators and the oracle's cycle (CPU).
+
+The operators (oper::setupPMG, src/operators.cpp:1553-1634) are pinned by
+interpolation exactness; the cycle (multiGrid::cycle, src/multigrid.cpp:169-313)
+by its small-time-step limit: with forward-Euler smoothing, one smoothing step
+down and one up per level and a fixed dt, the full-approximation source makes
+every coarse level start from the restricted fine residual, so the fine
+correction per unit dt tends to
+
+    -2 * sum_l Pro(l -> fine) [ R(fine -> l) divF_fine / detJ_l ]
+
+as dt -> 0 (Richardson-extrapolated below).  A dropped or sign-flipped source
+term, a restriction of U instead of divF, a missing 1/detJ or a wrong smoothing
+count each change that limit at O(1).
+"""
+import copy
+
+import numpy as np
+import pytest
+
+from flurrypp_amd import cases
+from flurrypp_amd.mesh import pmg_operators, pts1d, ref_points
+from flurrypp_amd.solver import HostSetup
+from oracle.oracle import Oracle, pmg_cycle
+
+
+def _poly(x, degs):
+    """Tensor polynomial with fixed coefficients, total degree <= degs per axis."""
+    v = np.ones(len(x))
+    for d in range(x.shape[1]):
+        c = np.arange(1, degs + 2) * (0.3 + 0.1 * d)
+        v = v * np.polynomial.polynomial.polyval(x[:, d], c)
+    return v
+
+
+@pytest.mark.parametrize("D", [2, 3])
+@pytest.mark.parametrize("P", [1, 2, 3, 4])
+def test_transfer_operators_interpolate_exactly(D, P):
+    pro, res = pmg_operators(D, P)
+    xs, _, _ = ref_points(D, P)
+    xf, _, _ = ref_points(D, P + 1)
+    xc, _, _ = ref_points(D, P - 1)
+    u = _poly(xs, P)
+    np.testing.assert_allclose(pro @ u, _poly(xf, P), rtol=0, atol=1e-12)
+    np.testing.assert_allclose(res @ u, _poly(xc, P), rtol=0, atol=1e-12)
+    # restriction after prolongation is the identity on degree-P data
+    _, res_up = pmg_operators(D, P + 1)
+    np.testing.assert_allclose(res_up @ pro, np.eye(len(xs)), rtol=0, atol=1e-12)
+
+
+def test_transfer_operators_tensor_structure():
+    """2-D operators are Kronecker products of the 1-D Lagrange matrices, x fastest."""
+    P = 3
+    s = pts1d(P)
+    t = pts1d(P + 1)
+    L = np.array([[np.prod([(y - s[m]) / (s[i] - s[m]) for m in range(P + 1) if m != i]) for i in range(P + 1)]
+                  for y in t])
+    pro, _ = pmg_operators(2, P)
+    np.testing.assert_allclose(pro, np.kron(L, L), rtol=0, atol=1e-14)
+    assert pmg_operators(2, 0)[1] is None
+
+
+def _levels(setup, low):
+    out = []
+    for q in range(low, setup.order):
+        pc = copy.copy(setup.params)
+        pc.order = q
+        out.append(HostSetup(pc, setup.mesh))
+    return out + [setup]
+
+
+def _cycle_increment(order, low, dt, U0):
+    s, _ = cases.bump2d(nx=6, ny=3, order=order, mach=0.3, dt_type=0, time_type=0, dt=dt)
+    lv = _levels(s, low)
+    ors = [Oracle.from_setup(h) for h in lv]
+    for h, o in zip(lv, ors):
+        o.initializeSolution(h.initial_condition(), 0)
+    ors[-1].set_U(U0)
+    R = [pmg_operators(2, h.order)[1] for h in lv[1:]]
+    Pr = [pmg_operators(2, h.order)[0] for h in lv[:-1]]
+    pmg_cycle(ors, R, Pr, 1)
+    return (ors[-1].get_U() - U0) / dt, lv, R, Pr
+
+
+@pytest.mark.parametrize("order,low", [(2, 1), (3, 1), (2, 0)])
+def test_cycle_small_dt_limit(order, low):
+    s, U = cases.bump2d(nx=6, ny=3, order=order, mach=0.3, dt_type=0, time_type=0)
+    rng = np.random.default_rng(5)
+    U0 = U * (1 + 0.01 * rng.standard_normal(U.shape))
+    fine = Oracle.from_setup(s)
+    fine.set_U(U0)
+    fine.calcResidual(0)
+    S, E, N = U0.shape
+    dF = fine.get("divF_spts", 0).reshape(S, E, N)
+    a, lv, R, Pr = _cycle_increment(order, low, 1e-4, U0)
+    b, _, _, _ = _cycle_increment(order, low, 5e-5, U0)
+    rich = 2 * b - a
+    # closed form: every level l contributes -2 Pro(l->fine) R(fine->l) dF / J_l
+    lim = np.zeros_like(U0)
+    nl = len(lv)
+    for l in range(nl - 1):
+        r = dF
+        for i in range(nl - 2, l - 1, -1):          # restrict fine -> l
+            r = (R[i] @ r.reshape(r.shape[0], -1)).reshape(R[i].shape[0], E, N)
+        c = -2.0 * r / lv[l].geo.detjac_spts[..., None]
+        for i in range(l, nl - 1):                  # prolong l -> fine
+            c = (Pr[i] @ c.reshape(c.shape[0], -1)).reshape(Pr[i].shape[0], E, N)
+        lim += c
+    err = np.linalg.norm(rich - lim) / np.linalg.norm(lim)
+    assert err < 1e-5, err
+    # and the raw increment is O(dt) away from it, so the test can tell
+    assert np.linalg.norm(a - lim) / np.linalg.norm(lim) > 10 * err
+
+
+def test_cycle_accelerates_steady_convergence():
+    """Bump P=2 with levels 0..2, RK44, local CFL: the cycle drives the
+    residual far below plain RK in the same number of fine iterations."""
+    s, U = cases.bump2d(nx=8, ny=4, order=2, mach=0.3)
+    lv = _levels(s, 0)
+    ors = [Oracle.from_setup(h) for h in lv]
+    for h, o in zip(lv, ors):
+        o.initializeSolution(h.initial_condition(), h.params.dtType)
+    R = [pmg_operators(2, h.order)[1] for h in lv[1:]]
+    Pr = [pmg_operators(2, h.order)[0] for h in lv[:-1]]
+    plain = Oracle.from_setup(s)
+    plain.initializeSolution(U, s.params.dtType)
+    for _ in range(400):
+        ors[-1].update(1)
+        pmg_cycle(ors, R, Pr, 1)
+        plain.update(1)
+    r_mg, r_plain = ors[-1].residual_norm(2)[0], plain.residual_norm(2)[0]
+    assert np.isfinite(r_mg) and r_mg < 0.1 * r_plain, (r_mg, r_plain)
