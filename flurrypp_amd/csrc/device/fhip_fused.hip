@@ -751,7 +751,7 @@ static void ensure_work(Ctx& c) {
 
 bool fused_setup(Ctx& c, const fhip_desc* d) {
   const Sizes& z = c.z;
-  if (z.P < 1 || z.P > 5) return false;
+  if (z.P < 0 || z.P > 5) return false;
   std::vector<char> blob;
   size_t off[8];
   if (!build_tables(c, d, blob, off)) return false;
@@ -983,6 +983,7 @@ static Ops get_ops_r(const Ctx& c) {
   const int D = c.z.D, P = c.z.P;
   if (D == 2) {
     switch (P) {
+      case 0: return ops_for<2, 0, R>();
       case 1: return ops_for<2, 1, R>();
       case 2: return ops_for<2, 2, R>();
       case 3: return ops_for<2, 3, R>();
@@ -991,6 +992,7 @@ static Ops get_ops_r(const Ctx& c) {
     }
   }
   switch (P) {
+    case 0: return ops_for<3, 0, R>();
     case 1: return ops_for<3, 1, R>();
     case 2: return ops_for<3, 2, R>();
     case 3: return ops_for<3, 3, R>();

@@ -32,6 +32,8 @@ CASES = {
     "tgv3d_p2_euler_jitter": lambda: cases.tgv3d(n=4, order=2, viscous=0, jitter=0.1),
     "bump2d_p2_char_slip_local_dt": lambda: cases.bump2d(nx=12, ny=5, order=2),
     "bump2d_p3_ns_adiabatic": lambda: _bump_ns(),
+    "bump2d_p0_char_slip_local_dt": lambda: cases.bump2d(nx=12, ny=5, order=0),
+    "tgv3d_p0_ns_jitter": lambda: cases.tgv3d(n=4, order=0, viscous=1, jitter=0.1),
 }
 
 

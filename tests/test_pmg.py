@@ -130,3 +130,24 @@ def test_cycle_accelerates_steady_convergence():
         plain.update(1)
     r_mg, r_plain = ors[-1].residual_norm(2)[0], plain.residual_norm(2)[0]
     assert np.isfinite(r_mg) and r_mg < 0.1 * r_plain, (r_mg, r_plain)
+
+
+@pytest.mark.parametrize("make", [lambda: cases.bump2d(nx=6, ny=3, order=0),
+                                  lambda: cases.tgv3d(n=3, order=0, viscous=0, jitter=0.1)])
+def test_order0_is_finite_volume(make):
+    """At P=0 (the coarsest multigrid level) FR with the DG correction is the
+    first-order finite-volume scheme: divF_spts = 1/2 * sum over the element's
+    faces of the common normal flux times dA (the volume divergence vanishes
+    and the element's discontinuous normal fluxes sum to zero on
+    straight-sided elements)."""
+    s, U = make()
+    o = Oracle.from_setup(s)
+    o.initializeSolution(U * (1 + 0.01 * np.sin(np.arange(U.size)).reshape(U.shape)), s.params.dtType)
+    o.calcResidual(0)
+    E, N, F = o.E, o.N, o.F
+    assert o.S == 1
+    divF = o.get("divF_spts", 0).reshape(E, N)
+    Fn = o.get("Fn_fpts").reshape(F, E, N)
+    fv = 0.5 * Fn.sum(axis=0)
+    np.testing.assert_allclose(divF, fv, rtol=0, atol=1e-13 * np.abs(Fn).max())
+    assert np.abs(divF).max() > 1e-3 * np.abs(Fn).max()
