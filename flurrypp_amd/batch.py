@@ -11,13 +11,15 @@ flags.
 """
 from __future__ import annotations
 
+import copy
 import ctypes as C
+import time
 from dataclasses import dataclass
 
 import numpy as np
 
 from . import _native as nat
-from .mesh import Faces, Geometry, Operators
+from .mesh import Faces, Geometry, Operators, pmg_operators
 from .solver import HostSetup
 
 
@@ -49,9 +51,11 @@ class MergedSetup:
         for i, m in enumerate(members):
             self.ele_start[i + 1] = self.ele_start[i] + m.setup.nEles
         self.nEles = int(self.ele_start[-1])
-        cat1 = lambda name: np.ascontiguousarray(np.concatenate([getattr(m.setup.geo, name) for m in members], axis=1))  # noqa: E731
-        self.geo = Geometry(pos_spts=cat1("pos_spts"), pos_fpts=cat1("pos_fpts"), jginv_spts=cat1("jginv_spts"),
-                            detjac_spts=cat1("detjac_spts"), norm_fpts=cat1("norm_fpts"), da_fpts=cat1("da_fpts"))
+        # element axis: 1 for [pt][ele](...), 2 for JGinv_spts [dim][spt][ele][dim]
+        def cat(name, axis=1):
+            return np.ascontiguousarray(np.concatenate([getattr(m.setup.geo, name) for m in members], axis=axis))
+        self.geo = Geometry(pos_spts=cat("pos_spts"), pos_fpts=cat("pos_fpts"), jginv_spts=cat("jginv_spts", 2),
+                            detjac_spts=cat("detjac_spts"), norm_fpts=cat("norm_fpts"), da_fpts=cat("da_fpts"))
         eL, lL, eR, lR, rr, bc, gid, per, bst = [], [], [], [], [], [], [], [], []
         for i, m in enumerate(members):
             f = m.setup.faces
@@ -144,11 +148,17 @@ class BatchGroup:
 
 
 class BatchedSolver:
-    """Independent cases grouped by order; one context per order group."""
+    """Independent cases grouped by order; one context per order group.  With
+    pmg=True every group of order P > low_order also gets coarse groups of
+    orders low_order..P-1 (the same meshes) and a P-multigrid cycle after each
+    of its updates."""
 
-    def __init__(self, members: list[Member], path: int = nat.PATH_FUSED):
+    def __init__(self, members: list[Member], path: int = nat.PATH_FUSED, pmg: bool = False, low_order: int = 0,
+                 smooth_steps: int = 1):
         self.members = members
         self.path = path
+        self.pmg = pmg
+        self.smooth = smooth_steps
         groups: dict[tuple, list[int]] = {}
         for i, m in enumerate(members):
             groups.setdefault((m.setup.nDims, m.setup.order), []).append(i)
@@ -165,16 +175,63 @@ class BatchedSolver:
         for g in self.groups:
             g.initialize()
         self.converged = np.zeros(len(members), dtype=bool)
+        # P-multigrid: coarse groups per fine group, and one fhip_pmg each
+        self.coarse: list[list[BatchGroup]] = [[] for _ in self.groups]
+        self._mg: list = [None] * len(self.groups)
+        self._ops = []
+        if pmg:
+            for gi, g in enumerate(self.groups):
+                P = g.m.order
+                if P <= low_order:
+                    continue
+                for q in range(low_order, P):
+                    cm = []
+                    for m in g.m.members:
+                        pc = copy.copy(m.setup.params)
+                        pc.order = q
+                        hs = HostSetup(pc, m.setup.mesh)
+                        cm.append(Member(hs, hs.initial_condition(), m.tag))
+                    cg = BatchGroup(MergedSetup(cm))
+                    cg.initialize()
+                    self.coarse[gi].append(cg)
+                levels = self.coarse[gi] + [g]
+                D = g.m.nDims
+                sp = g.m.params.sptsTypeQuad
+                R = [pmg_operators(D, lv.m.order, sp)[1] for lv in levels[1:]]
+                Pr = [pmg_operators(D, lv.m.order, sp)[0] for lv in levels[:-1]]
+                self._ops.append((R, Pr))
+                n = len(levels)
+                hs_ = (C.c_void_p * n)(*[lv._ctx.value for lv in levels])
+                Ra = (nat._dp * (n - 1))(*[nat.dptr(a) for a in R])
+                Pa = (nat._dp * (n - 1))(*[nat.dptr(a) for a in Pr])
+                h = C.c_void_p()
+                nat.hip_check(nat.hip_lib().fhip_pmg_create(hs_, n, Ra, Pa, smooth_steps, C.byref(h)))
+                self._mg[gi] = h
 
     def close(self):
+        for h in self._mg:
+            if h is not None and h.value:
+                nat.hip_lib().fhip_pmg_destroy(h)
+        self._mg = [None] * len(self.groups)
+        for cs in self.coarse:
+            for g in cs:
+                g.close()
         for g in self.groups:
             g.close()
 
     def step(self, n: int = 1) -> None:
         arr = (C.c_void_p * len(self.groups))(*[g._ctx.value for g in self.groups])
-        nat.hip_check(nat.hip_lib().fhip_update_many(arr, len(self.groups), n, self.path))
+        lib = nat.hip_lib()
+        if not any(h is not None for h in self._mg):
+            nat.hip_check(lib.fhip_update_many(arr, len(self.groups), n, self.path))
+        else:
+            for _ in range(n):
+                nat.hip_check(lib.fhip_update_many(arr, len(self.groups), 1, self.path))
+                for h in self._mg:
+                    if h is not None:
+                        nat.hip_check(lib.fhip_pmg_cycle(h, self.path))
         for g in self.groups:
-            nat.hip_check(nat.hip_lib().fhip_sync(g._ctx))
+            nat.hip_check(lib.fhip_sync(g._ctx))
 
     def residuals(self, res_type: int = 2) -> list[np.ndarray]:
         per = [g.residuals(res_type) for g in self.groups]
@@ -189,12 +246,18 @@ class BatchedSolver:
         """Step until every member's residual (field `field`) is <= tol; freeze
         converged members.  Returns steps and residual per member."""
         steps = np.zeros(len(self.members), dtype=np.int64)
+        t_conv = np.full(len(self.members), np.nan)
         done = 0
         last = [None] * len(self.members)
+        dof = np.array([m.setup.nEles * m.setup.nSpts * m.setup.params.nRKSteps for m in self.members])
+        work = 0
+        t0 = time.perf_counter()
         while done < max_steps and not self.converged.all():
             self.step(check_every)
             done += check_every
+            work += int(dof[~self.converged].sum()) * check_every
             res = self.residuals(res_type)
+            now = time.perf_counter() - t0
             for i, r in enumerate(res):
                 if not self.converged[i]:
                     last[i] = r
@@ -203,8 +266,12 @@ class BatchedSolver:
                         raise RuntimeError(f"NaN in the residual of batch member {i}")
                     if r[field] <= tol:
                         self.converged[i] = True
+                        t_conv[i] = now
             for gi, g in enumerate(self.groups):
                 act = np.array([0 if self.converged[i] else 1 for i, (gg, _) in enumerate(self.index) if gg == gi],
                                dtype=np.int32)
                 g.set_active(act)
-        return {"steps": steps, "residual": last, "converged": self.converged.copy()}
+                for cg in self.coarse[gi]:
+                    cg.set_active(act)
+        return {"steps": steps, "residual": last, "converged": self.converged.copy(), "time_s": t_conv,
+                "wall_s": time.perf_counter() - t0, "dof_stages": work}

@@ -20,7 +20,7 @@ import numpy as np
 
 from . import _native as nat
 from . import cases
-from .solver import Solver
+from .solver import MultiGrid, Solver
 
 
 @dataclass
@@ -71,6 +71,7 @@ class Server:
     steps_per_tick: int = 10
     check_every: int = 50          # steady requests: residual check interval (steps)
     keep_results: bool = False
+    pmg: bool = False              # steady requests: P-multigrid down to order 0 after every step
     log: object = None
     active: list = field(default_factory=list)
     done: list = field(default_factory=list)
@@ -91,14 +92,19 @@ class Server:
             while waiting and len(self.active) < self.max_active:
                 r = waiting[0]
                 setup, U = build_case(r)
-                need = footprint_bytes(setup.nDims, setup.order, setup.nEles, bool(setup.params.viscous))
+                visc = bool(setup.params.viscous)
+                need = footprint_bytes(setup.nDims, setup.order, setup.nEles, visc)
+                if self.pmg and r.steady_tol is not None:
+                    need += sum(footprint_bytes(setup.nDims, q, setup.nEles, visc) for q in range(setup.order))
                 if used + need > self.hbm_budget and self.active:
                     break
                 waiting.pop(0)
                 s = Solver(setup.params, setup=setup)
                 s.initializeSolution(U)
+                mg = MultiGrid(s, low_order=0, smooth_steps=1) if (self.pmg and r.steady_tol is not None
+                                                                    and setup.order > 0) else None
                 r.admitted, r.dof = now(), setup.nEles * setup.nSpts
-                ctx[r.rid] = (s, need, setup.params.nRKSteps)
+                ctx[r.rid] = (s, need, setup.params.nRKSteps, mg)
                 used += need
                 self.active.append(r)
             if not self.active:
@@ -106,12 +112,20 @@ class Server:
                     time.sleep(max(0.0, min(0.01, pending[0].arrival - now())))
                 continue
             # one tick: every active request advances together
+            lib = nat.hip_lib()
             for path in (nat.PATH_FUSED, nat.PATH_MIXED):
                 group = [r for r in self.active if (r.precision == "mixed") == (path == nat.PATH_MIXED)]
                 if not group:
                     continue
                 arr = (C.c_void_p * len(group))(*[ctx[r.rid][0]._ctx.value for r in group])
-                nat.hip_check(nat.hip_lib().fhip_update_many(arr, len(group), self.steps_per_tick, path))
+                cyc = [ctx[r.rid][3] for r in group if ctx[r.rid][3] is not None]
+                if not cyc:
+                    nat.hip_check(lib.fhip_update_many(arr, len(group), self.steps_per_tick, path))
+                    continue
+                for _ in range(self.steps_per_tick):
+                    nat.hip_check(lib.fhip_update_many(arr, len(group), 1, path))
+                    for mg in cyc:
+                        mg.cycle(path, sync=False)
             for r in self.active:
                 ctx[r.rid][0].sync()
                 r.steps += self.steps_per_tick
@@ -119,7 +133,7 @@ class Server:
             # completion and eviction
             still = []
             for r in self.active:
-                s, need, _ = ctx[r.rid]
+                s, need, _, mg = ctx[r.rid]
                 fin = False
                 if r.steady_tol is not None:
                     if r.steps % self.check_every == 0 or r.steps >= r.max_steps:
@@ -134,6 +148,8 @@ class Server:
                     r.finished = now()
                     if self.keep_results:
                         r.result = s.get_U()
+                    if mg is not None:
+                        mg.close()
                     s.close()
                     used -= need
                     del ctx[r.rid]
