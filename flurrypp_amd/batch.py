@@ -159,9 +159,12 @@ class BatchedSolver:
         self.path = path
         self.pmg = pmg
         self.smooth = smooth_steps
+        # one merged context per element type, order and solver options
         groups: dict[tuple, list[int]] = {}
         for i, m in enumerate(members):
-            groups.setdefault((m.setup.nDims, m.setup.order), []).append(i)
+            q = m.setup.params
+            key = (m.setup.nDims, m.setup.order, q.equation, q.viscous, q.riemannType, q.timeType, q.dtType, q.gamma)
+            groups.setdefault(key, []).append(i)
         self.index = []  # (group, position) of each member
         self.groups: list[BatchGroup] = []
         pos = [None] * len(members)

@@ -49,6 +49,7 @@ T* dalloc(size_t n) {
   void* p = nullptr;
   if (n == 0) n = 1;
   HCHK(hipMalloc(&p, n * sizeof(T)));
+  fhip::poison_fill(p, n * sizeof(T));
   return (T*)p;
 }
 

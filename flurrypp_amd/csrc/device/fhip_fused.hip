@@ -247,10 +247,12 @@ __global__ __launch_bounds__(256) void k_volume(int E, const double* __restrict_
                                                  Tabs tb, Phys ph) {
   using Sh = Shape<D, P>;
   constexpr int S = Sh::S, F = Sh::F, N = Sh::N, EB = Sh::EB, P1 = Sh::P1, NC = Sh::NC;
-  // sF holds the physical gradient (viscous runs) until the gradient has been
-  // extrapolated to the faces, then the transformed flux at the same indices.
+  // sF holds, in turn (viscous runs): dUc at the flux points, the physical
+  // gradient until it has been extrapolated to the faces, then the transformed
+  // flux at the same indices as the gradient.
+  constexpr int SF = (VISC && F > D * S) ? F : D * S;
   __shared__ R sU[EB * N * S];
-  __shared__ R sF[EB * D * N * S];
+  __shared__ R sF[EB * N * SF];
   R* const sG = sF;
   const int e0 = blockIdx.x * EB, nb = min(EB, E - e0);
   {
@@ -265,8 +267,14 @@ __global__ __launch_bounds__(256) void k_volume(int E, const double* __restrict_
     // phase 1: reference-space gradient (opp_grad_spts) plus correction
     // (opp_correctU), then the physical gradient (correctGradU transform,
     // src/solver.cpp:1105-1136), kept in LDS for phase 2 and the faces
+    R* const sC = sF;
+    {
+      const long long o = (long long)e0 * N * F;
+      for (int i = threadIdx.x; i < nb * N * F; i += blockDim.x) sC[i] = dUce[o + i];
+    }
+    __syncthreads();
+    R gr[D][N];
     if (act) {
-      R gr[D][N];
 #pragma unroll
       for (int d = 0; d < D; d++) {
         const int* gi = tb.grd_idx + (d * S + s) * P1;
@@ -279,10 +287,13 @@ __global__ __launch_bounds__(256) void k_volume(int E, const double* __restrict_
           for (int m = 0; m < P1; m++) a += R(gw[m]) * sU[(b * N + k) * S + gi[m]];
           R c = 0;
 #pragma unroll
-          for (int j = 0; j < NC; j++) c += R(cw[j]) * dUce[(e * N + k) * F + tb.cor_idx[s * NC + j]];
+          for (int j = 0; j < NC; j++) c += R(cw[j]) * sC[(b * N + k) * F + tb.cor_idx[s * NC + j]];
           gr[d][k] = c + a;
         }
       }
+    }
+    __syncthreads();  // sC is overwritten by the gradient below
+    if (act) {
       R G[D][D];
 #pragma unroll
       for (int r = 0; r < D; r++)
@@ -468,8 +479,14 @@ __global__ __launch_bounds__(256) void k_rk(int E, int mode, double a, double bc
   using Sh = Shape<D, P>;
   constexpr int S = Sh::S, F = Sh::F, N = Sh::N, EB = Sh::EB, P1 = Sh::P1, NC = Sh::NC;
   __shared__ R sU[EB * N * S];
+  __shared__ R sD[EB * N * F];  // Fn - disFn at the block's flux points
   const int e0 = blockIdx.x * EB, nb = min(EB, E - e0);
   const int b = threadIdx.x / S, s = threadIdx.x - b * S;
+  {
+    const long long o = (long long)e0 * N * F;
+    for (int i = threadIdx.x; i < nb * N * F; i += blockDim.x) sD[i] = Fne[o + i] - disFne[o + i];
+  }
+  __syncthreads();
   if (mode == 5) {
     // residual only (calcResidual(0) + correctDivFlux): divF_spts[0], no update
     if (b < nb) {
@@ -478,10 +495,7 @@ __global__ __launch_bounds__(256) void k_rk(int E, int mode, double a, double bc
       for (int k = 0; k < N; k++) {
         R c = 0;
 #pragma unroll
-        for (int j = 0; j < NC; j++) {
-          const long long jf = (e * N + k) * F + tb.cor_idx[s * NC + j];
-          c += R(tb.cor_w[s * NC + j]) * (Fne[jf] - disFne[jf]);
-        }
+        for (int j = 0; j < NC; j++) c += R(tb.cor_w[s * NC + j]) * sD[(b * N + k) * F + tb.cor_idx[s * NC + j]];
         divF0e[(e * N + k) * S + s] = double(c + divFe[(e * N + k) * S + s]);
       }
     }
@@ -498,10 +512,7 @@ __global__ __launch_bounds__(256) void k_rk(int E, int mode, double a, double bc
       const long long iu = (e * N + k) * S + s;
       R c = 0;
 #pragma unroll
-      for (int j = 0; j < NC; j++) {
-        const long long jf = (e * N + k) * F + tb.cor_idx[s * NC + j];
-        c += R(tb.cor_w[s * NC + j]) * (Fne[jf] - disFne[jf]);
-      }
+      for (int j = 0; j < NC; j++) c += R(tb.cor_w[s * NC + j]) * sD[(b * N + k) * F + tb.cor_idx[s * NC + j]];
       const double r0 = double(c + divFe[iu]);
       if (divF0e) divF0e[iu] = r0;
       // P-multigrid source: divF_spts[step] + src_spts (src/solver.cpp:480,514)
@@ -661,6 +672,7 @@ template <class T>
 static T* dal(size_t n) {
   void* p = nullptr;
   FCHK(hipMalloc(&p, (n ? n : 1) * sizeof(T)));
+  poison_fill(p, (n ? n : 1) * sizeof(T));
   return (T*)p;
 }
 
@@ -869,6 +881,7 @@ int fused_n_meshes(Ctx& c) { return fs(c).nMesh; }
 
 void fused_set_active(Ctx& c, const int* active) {
   FusedState& st = fs(c);
+  if (!st.active || !st.eleMesh) c.drop_graphs();  // captured steps hold null flag pointers
   if (!st.active) st.active = dal<int>(st.nMesh);
   if (!st.eleMesh) {  // single mesh: every element belongs to mesh 0
     std::vector<int> em(c.z.E, 0);
@@ -1019,7 +1032,11 @@ void fused_begin(Ctx& c, bool mixed, bool src) {
   st.use_src = src;
   if (src) {
     if (!c.src) throw std::invalid_argument("no P-multigrid source term on this context");
-    if (!st.srce) FCHK(hipMalloc(&st.srce, nS * sizeof(double)));
+    if (!st.srce) {
+      c.drop_graphs();
+      FCHK(hipMalloc(&st.srce, nS * sizeof(double)));
+      poison_fill(st.srce, nS * sizeof(double));
+    }
     hipLaunchKernelGGL(k_to_em, dim3(nblk(nS, 256)), dim3(256), 0, c.stream, nS, z.S, z.E, z.N, c.src, st.srce);
   }
   long long nW = (long long)z.E * z.F;

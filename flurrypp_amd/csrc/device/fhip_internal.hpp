@@ -4,6 +4,8 @@
 
 #include <hip/hip_runtime.h>
 
+#include <cstdlib>
+
 #include <map>
 #include <string>
 #include <vector>
@@ -37,6 +39,22 @@ struct Timer {
   size_t used = 0;
 };
 
+// Debug aid: with FHIP_POISON_ALLOC=1 in the environment every device
+// allocation of the library is filled with 0xFF bytes (NaN in FP64/FP32, -1 in
+// int32), so that a read of memory no kernel has written shows up as NaN.
+inline bool poison_alloc() {
+  static const bool on = [] {
+    const char* v = std::getenv("FHIP_POISON_ALLOC");
+    return v && v[0] == '1';
+  }();
+  return on;
+}
+inline void poison_fill(void* p, size_t bytes) {
+  if (!(poison_alloc() && p && bytes)) return;
+  (void)hipMemset(p, 0xFF, bytes);
+  (void)hipDeviceSynchronize();
+}
+
 struct Ctx {
   Sizes z;
   fhip_params prm;
@@ -66,8 +84,21 @@ struct Ctx {
   bool timing = false;
   Timer timer;
   hipEvent_t ev0 = nullptr, ev1 = nullptr;
-  // captured RK steps (fused path): index 0 without, 1 with wave-speed refresh
+  // captured RK steps (fused path): [with P-multigrid source][FP64, mixed].
+  // A graph holds the device pointers of the moment it was captured: drop
+  // them whenever an array they use is (re)allocated.
   hipGraphExec_t graph[2][3] = {{nullptr, nullptr, nullptr}, {nullptr, nullptr, nullptr}};
+  void drop_graphs() {
+    bool any = false;
+    for (auto& row : graph)
+      for (auto& g : row) any = any || g != nullptr;
+    if (any) (void)hipStreamSynchronize(stream);  // no replay still in flight
+    for (auto& row : graph)
+      for (auto& g : row) {
+        if (g) (void)hipGraphExecDestroy(g);
+        g = nullptr;
+      }
+  }
   bool fused_ok = false;
   bool use_graphs = true;
   bool batched = false;  // several meshes or a boundary-state table

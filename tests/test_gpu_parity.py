@@ -34,6 +34,9 @@ CASES = {
     "bump2d_p3_ns_adiabatic": lambda: _bump_ns(),
     "bump2d_p0_char_slip_local_dt": lambda: cases.bump2d(nx=12, ny=5, order=0),
     "tgv3d_p0_ns_jitter": lambda: cases.tgv3d(n=4, order=0, viscous=1, jitter=0.1),
+    "vortex2d_p5_jitter": lambda: cases.vortex2d(n=4, order=5),
+    "bump2d_p5_char_slip_local_dt": lambda: cases.bump2d(nx=6, ny=4, order=5),
+    "tgv3d_p5_ns_jitter": lambda: cases.tgv3d(n=2, order=5, viscous=1, jitter=0.1),
 }
 
 
