@@ -55,9 +55,11 @@ def cpu_baseline(n: int, steps: int) -> dict:
 
 def batched_form(n_cases: int, seed: int, tol: float, max_steps: int, nx_hi: int, ny_hi: int) -> dict:
     """Independent 2-D Euler Gaussian-bump channels (sizes, orders 1..5 and
-    inlet Mach 0.2-0.5 drawn seeded), merged per order, local-CFL RK44 with a
-    P-multigrid cycle down to order 0 after every step, run until each case's
-    L2 density residual is <= tol (converged cases are frozen)."""
+    inlet Mach 0.2-0.5 drawn seeded), merged per order, with the settings of
+    FlurryPP's own P-multigrid channel case (tests/euler/channel/input_channelPMG:
+    tuned steady RK (timeType 6), local CFL 0.9, lowOrder 0, smoothSteps 2),
+    run until each case's L2 density residual is <= tol; converged and
+    diverged cases are frozen, the rest stop at max_steps."""
     from flurrypp_amd import cases
     from flurrypp_amd.batch import BatchedSolver, Member
     rng = np.random.default_rng(seed)
@@ -65,20 +67,26 @@ def batched_form(n_cases: int, seed: int, tol: float, max_steps: int, nx_hi: int
     for i in range(n_cases):
         nx, ny = int(rng.integers(16, nx_hi + 1)), int(rng.integers(8, ny_hi + 1))
         P, M = int(rng.integers(1, 6)), float(rng.uniform(0.2, 0.5))
-        s, U = cases.bump2d(nx=nx, ny=ny, order=P, mach=M, seed=seed * 100003 + i)
+        s, U = cases.bump2d(nx=nx, ny=ny, order=P, mach=M, seed=seed * 100003 + i, time_type=6, cfl=0.9)
         ms.append(Member(s, U, (nx, ny, P, M)))
     t = time.perf_counter()
-    b = BatchedSolver(ms, pmg=True, low_order=0)
+    b = BatchedSolver(ms, pmg=True, low_order=0, smooth_steps=2)
     setup_s = time.perf_counter() - t
-    out = b.solve_steady(tol=tol, max_steps=max_steps, check_every=50)
+    out = b.solve_steady(tol=tol, max_steps=max_steps, check_every=50, stop_on_nan=False)
     b.close()
-    conv = out["converged"]
+    conv, div = out["converged"], out["diverged"]
+    by_order = {P: {"cases": int(sum(1 for m in ms if m.tag[2] == P)),
+                    "converged": int(sum(1 for i, m in enumerate(ms) if m.tag[2] == P and conv[i])),
+                    "diverged": int(sum(1 for i, m in enumerate(ms) if m.tag[2] == P and div[i]))}
+                for P in range(1, 6)}
     tc = out["time_s"][conv]
     dof = sum(m.setup.nEles * m.setup.nSpts for m in ms)
     return {
         "workload": f"{n_cases} 2-D Euler Gaussian-bump channels, quads 16..{nx_hi} x 8..{ny_hi}, P 1..5, "
-                    f"Mach 0.2-0.5, local CFL, P-multigrid (orders 0..P-1), steady to L2 residual {tol:g}",
-        "cases": n_cases, "dof": dof, "converged": int(conv.sum()), "wall_s": round(out["wall_s"], 3),
+                    f"Mach 0.2-0.5, timeType 6, local CFL 0.9, P-multigrid (orders 0..P-1, 2 smoothing steps), "
+                    f"steady to L2 residual {tol:g} within {max_steps} steps",
+        "cases": n_cases, "dof": dof, "converged": int(conv.sum()), "diverged": int(div.sum()),
+        "by_order": by_order, "wall_s": round(out["wall_s"], 3),
         "p50_s": float(np.percentile(tc, 50)) if len(tc) else None,
         "p99_s": float(np.percentile(tc, 99)) if len(tc) else None,
         "steps_p50": int(np.percentile(out["steps"], 50)), "steps_max": int(out["steps"].max()),

@@ -245,36 +245,43 @@ class BatchedSolver:
         return [per[gi][j] for gi, j in self.index]
 
     def solve_steady(self, tol: float = 1e-10, max_steps: int = 100000, check_every: int = 50,
-                     res_type: int = 2, field: int = 0) -> dict:
+                     res_type: int = 2, field: int = 0, stop_on_nan: bool = True) -> dict:
         """Step until every member's residual (field `field`) is <= tol; freeze
-        converged members.  Returns steps and residual per member."""
+        converged members.  A member whose residual turns NaN raises, or with
+        stop_on_nan=False is frozen and reported in "diverged".  Returns steps,
+        residual and convergence time per member."""
         steps = np.zeros(len(self.members), dtype=np.int64)
         t_conv = np.full(len(self.members), np.nan)
+        diverged = np.zeros(len(self.members), dtype=bool)
         done = 0
         last = [None] * len(self.members)
         dof = np.array([m.setup.nEles * m.setup.nSpts * m.setup.params.nRKSteps for m in self.members])
         work = 0
         t0 = time.perf_counter()
-        while done < max_steps and not self.converged.all():
+        while done < max_steps and not (self.converged | diverged).all():
             self.step(check_every)
             done += check_every
-            work += int(dof[~self.converged].sum()) * check_every
+            work += int(dof[~(self.converged | diverged)].sum()) * check_every
             res = self.residuals(res_type)
             now = time.perf_counter() - t0
             for i, r in enumerate(res):
-                if not self.converged[i]:
+                if not (self.converged[i] or diverged[i]):
                     last[i] = r
                     steps[i] = done
-                    if np.isnan(r).any():
-                        raise RuntimeError(f"NaN in the residual of batch member {i}")
-                    if r[field] <= tol:
+                    if not np.isfinite(r).all():
+                        if stop_on_nan:
+                            raise RuntimeError(f"NaN in the residual of batch member {i}")
+                        diverged[i] = True
+                    elif r[field] <= tol:
                         self.converged[i] = True
                         t_conv[i] = now
+            stopped = self.converged | diverged
             for gi, g in enumerate(self.groups):
-                act = np.array([0 if self.converged[i] else 1 for i, (gg, _) in enumerate(self.index) if gg == gi],
+                act = np.array([0 if stopped[i] else 1 for i, (gg, _) in enumerate(self.index) if gg == gi],
                                dtype=np.int32)
                 g.set_active(act)
                 for cg in self.coarse[gi]:
                     cg.set_active(act)
-        return {"steps": steps, "residual": last, "converged": self.converged.copy(), "time_s": t_conv,
+        return {"steps": steps, "residual": last, "converged": self.converged.copy(), "diverged": diverged,
+                "time_s": t_conv,
                 "wall_s": time.perf_counter() - t0, "dof_stages": work}

@@ -56,17 +56,34 @@ struct Shape {
   static constexpr int NT = 256;
 };
 
-// Sparse operator tables (device pointers into one allocation).
+// Sparse operator tables (device pointers into one allocation).  Indices of
+// solution points are positions in the swizzled LDS layout (spt_lds below).
 struct Tabs {
-  const int* ext_idx;   // [F][P1]     solution points feeding flux point f
-  const double* ext_w;  // [F][P1]     opp_spts_to_fpts entries
-  const double* efn_w;  // [F][D][P1]  opp_extrapolate_fn entries (same points)
+  const int* ext_idx;   // [F][P1]    solution points feeding flux point f
+  const double* ext_w;  // [F][P1]    opp_spts_to_fpts entries
+  const int* efn_dir;   // [F]        the one direction d with opp_extrapolate_fn[d][f] != 0
+  const double* efn_w;  // [F][P1]    opp_extrapolate_fn[efn_dir[f]] entries (same points)
   const int* grd_idx;   // [D][S][P1]
-  const double* grd_w;  // [D][S][P1]  opp_grad_spts entries
-  const int* cor_idx;   // [S][NC]     flux points correcting solution point s
-  const double* cor_w;  // [S][NC]     opp_correction entries
-  const double* cu_w;   // [D][S][NC]  opp_correct_u entries
+  const double* grd_w;  // [D][S][P1] opp_grad_spts entries
+  const int* cor_idx;   // [S][NC]    flux points correcting solution point s
+  const double* cor_w;  // [S][NC]    opp_correction entries
+  const int* cu_idx;    // [D][S][2]  flux points with opp_correct_u[d][s][f] != 0
+  const double* cu_w;   // [D][S][2]  their opp_correct_u entries
 };
+
+// Position of solution point s = i + P1*(j + P1*k) in an element's LDS slab.
+// Each (j,k) row is rotated by j+k and each k plane by k, so that the P1^2
+// points a face's flux points read at one depth fall into distinct LDS banks
+// for all three face orientations (row-major order would give 4-way bank
+// conflicts on the x and y faces at P=3).  A bijection on every plane.
+__host__ __device__ constexpr int spt_lds(int D, int P1, int s) {
+  if (D == 2) {
+    const int i = s % P1, j = s / P1;
+    return j * P1 + (i + j) % P1;
+  }
+  const int i = s % P1, j = (s / P1) % P1, k = s / (P1 * P1);
+  return (k * P1 + (j + k) % P1) * P1 + (i + k) % P1;
+}
 
 // Face and volume intermediates of one precision.
 template <class R>
@@ -147,7 +164,10 @@ __global__ __launch_bounds__(256) void k_extrap(int E, const double* __restrict_
   __shared__ R sU[EB * N * S];
   const int e0 = blockIdx.x * EB, nb = min(EB, E - e0);
   const double* src = Ue + (long long)e0 * N * S;
-  for (int i = threadIdx.x; i < nb * N * S; i += blockDim.x) sU[i] = R(src[i]);
+  for (int i = threadIdx.x; i < nb * N * S; i += blockDim.x) {
+    const int r = i / S, q = i - r * S;
+    sU[r * S + spt_lds(D, P1, q)] = R(src[i]);
+  }
   __syncthreads();
   for (int i = threadIdx.x; i < nb * F; i += blockDim.x) {
     int b = i / F, f = i - b * F;
@@ -257,10 +277,14 @@ __global__ __launch_bounds__(256) void k_volume(int E, const double* __restrict_
   const int e0 = blockIdx.x * EB, nb = min(EB, E - e0);
   {
     const double* src = Ue + (long long)e0 * N * S;
-    for (int i = threadIdx.x; i < nb * N * S; i += blockDim.x) sU[i] = R(src[i]);
+    for (int i = threadIdx.x; i < nb * N * S; i += blockDim.x) {
+      const int r = i / S, q = i - r * S;
+      sU[r * S + spt_lds(D, P1, q)] = R(src[i]);
+    }
   }
   __syncthreads();
   const int b = threadIdx.x / S, s = threadIdx.x - b * S;
+  const int sw = spt_lds(D, P1, s);
   const bool act = b < nb;
   const long long e = e0 + b;
   if constexpr (VISC) {
@@ -279,15 +303,17 @@ __global__ __launch_bounds__(256) void k_volume(int E, const double* __restrict_
       for (int d = 0; d < D; d++) {
         const int* gi = tb.grd_idx + (d * S + s) * P1;
         const double* gw = tb.grd_w + (d * S + s) * P1;
-        const double* cw = tb.cu_w + (d * S + s) * NC;
+        const int* ci = tb.cu_idx + (d * S + s) * 2;
+        const double* cw = tb.cu_w + (d * S + s) * 2;
 #pragma unroll 1
         for (int k = 0; k < N; k++) {
           R a = 0;
 #pragma unroll
           for (int m = 0; m < P1; m++) a += R(gw[m]) * sU[(b * N + k) * S + gi[m]];
+          // the other correction entries of the row are exactly zero
           R c = 0;
 #pragma unroll
-          for (int j = 0; j < NC; j++) c += R(cw[j]) * sC[(b * N + k) * F + tb.cor_idx[s * NC + j]];
+          for (int j = 0; j < 2; j++) c += R(cw[j]) * sC[(b * N + k) * F + ci[j]];
           gr[d][k] = c + a;
         }
       }
@@ -306,7 +332,7 @@ __global__ __launch_bounds__(256) void k_volume(int E, const double* __restrict_
         for (int x = 0; x < D; x++) {
           R acc = gr[0][k] * G[0][x] + gr[1][k] * G[1][x];
           if constexpr (D == 3) acc += gr[2][k] * G[2][x];
-          sG[((b * D + x) * N + k) * S + s] = inv * acc;
+          sG[((b * D + x) * N + k) * S + sw] = inv * acc;
         }
     }
     __syncthreads();
@@ -332,14 +358,14 @@ __global__ __launch_bounds__(256) void k_volume(int E, const double* __restrict_
     // phase 2: flux at the point, transformed to the reference element
     R u[N];
 #pragma unroll
-    for (int k = 0; k < N; k++) u[k] = sU[(b * N + k) * S + s];
+    for (int k = 0; k < N; k++) u[k] = sU[(b * N + k) * S + sw];
     R Fl[D][N];
     if constexpr (VISC) {
       R g[D][N];
 #pragma unroll
       for (int x = 0; x < D; x++)
 #pragma unroll
-        for (int k = 0; k < N; k++) g[x][k] = sG[((b * D + x) * N + k) * S + s];
+        for (int k = 0; k < N; k++) g[x][k] = sG[((b * D + x) * N + k) * S + sw];
       R Fi[D][N];
       viscous_flux<D, R>(u, g, Fl, ph);
       inviscid_flux<D, R>(u, Fi, R(ph.gamma));
@@ -362,7 +388,7 @@ __global__ __launch_bounds__(256) void k_volume(int E, const double* __restrict_
         R acc = 0;
 #pragma unroll
         for (int d2 = 0; d2 < D; d2++) acc += G[d1][d2] * Fl[d2][k];
-        sF[((b * D + d1) * N + k) * S + s] = acc;
+        sF[((b * D + d1) * N + k) * S + sw] = acc;
       }
   }
   __syncthreads();
@@ -388,18 +414,16 @@ __global__ __launch_bounds__(256) void k_volume(int E, const double* __restrict_
     const int bb = i / F, f = i - bb * F;
     const long long ee = e0 + bb;
     const int* xi = tb.ext_idx + f * P1;
+    // extrapolateNormalFlux: of the sum over directions only the face's own
+    // normal direction is non-zero (the other terms are exact zeros)
+    const int dn = tb.efn_dir[f];
+    const double* ew = tb.efn_w + f * P1;
 #pragma unroll
     for (int k = 0; k < N; k++) {
-      R tot = 0;
+      R a = 0;
 #pragma unroll
-      for (int d = 0; d < D; d++) {
-        const double* ew = tb.efn_w + (f * D + d) * P1;
-        R a = 0;
-#pragma unroll
-        for (int m = 0; m < P1; m++) a += R(ew[m]) * sF[((bb * D + d) * N + k) * S + xi[m]];
-        tot = (d == 0) ? a : a + tot;
-      }
-      disFne[(ee * N + k) * F + f] = tot;
+      for (int m = 0; m < P1; m++) a += R(ew[m]) * sF[((bb * D + dn) * N + k) * S + xi[m]];
+      disFne[(ee * N + k) * F + f] = a;
     }
   }
 }
@@ -535,7 +559,7 @@ __global__ __launch_bounds__(256) void k_rk(int E, int mode, double a, double bc
       }
       if (frozen) un = u;
       Ue[iu] = un;
-      sU[(b * N + k) * S + s] = R(un);
+      sU[(b * N + k) * S + spt_lds(D, P1, s)] = R(un);
     }
   }
   __syncthreads();
@@ -678,54 +702,70 @@ static T* dal(size_t n) {
 
 // Extract the non-zero structure of the dense operators.  Returns false if an
 // operator row has more non-zeros than the tensor-product structure allows.
-static bool build_tables(const Ctx& c, const fhip_desc* d, std::vector<char>& blob, size_t off[8]) {
+static bool build_tables(const Ctx& c, const fhip_desc* d, std::vector<char>& blob, size_t off[10]) {
   const int D = c.z.D, S = c.z.S, F = c.z.F, P1 = c.z.P + 1, NC = 2 * D;
-  std::vector<int> ext_idx((size_t)F * P1, 0), grd_idx((size_t)D * S * P1, 0), cor_idx((size_t)S * NC, 0);
-  std::vector<double> ext_w((size_t)F * P1, 0.), efn_w((size_t)F * D * P1, 0.), grd_w((size_t)D * S * P1, 0.),
-      cor_w((size_t)S * NC, 0.), cu_w((size_t)D * S * NC, 0.);
+  std::vector<int> ext_idx((size_t)F * P1, 0), efn_dir(F, 0), grd_idx((size_t)D * S * P1, 0),
+      cor_idx((size_t)S * NC, 0), cu_idx((size_t)D * S * 2, 0);
+  std::vector<double> ext_w((size_t)F * P1, 0.), efn_w((size_t)F * P1, 0.), grd_w((size_t)D * S * P1, 0.),
+      cor_w((size_t)S * NC, 0.), cu_w((size_t)D * S * 2, 0.);
   for (int f = 0; f < F; f++) {
+    // the face's normal direction: the only d with non-zero extrapolate_fn entries
+    int dn = -1;
+    for (int dd = 0; dd < D; dd++)
+      for (int q = 0; q < S; q++)
+        if (d->opp_extrapolate_fn[((size_t)dd * F + f) * S + q] != 0.) {
+          if (dn >= 0 && dn != dd) return false;
+          dn = dd;
+        }
+    efn_dir[f] = dn < 0 ? 0 : dn;
     int n = 0;
-    for (int s = 0; s < S; s++) {
-      double v = d->opp_spts_to_fpts[(size_t)f * S + s];
-      bool nz = v != 0.;
-      for (int dd = 0; dd < D; dd++) nz = nz || d->opp_extrapolate_fn[((size_t)dd * F + f) * S + s] != 0.;
-      if (!nz) continue;
+    for (int q = 0; q < S; q++) {
+      double v = d->opp_spts_to_fpts[(size_t)f * S + q];
+      double w = dn < 0 ? 0. : d->opp_extrapolate_fn[((size_t)dn * F + f) * S + q];
+      if (v == 0. && w == 0.) continue;
       if (n >= P1) return false;
-      ext_idx[(size_t)f * P1 + n] = s;
+      ext_idx[(size_t)f * P1 + n] = spt_lds(D, P1, q);
       ext_w[(size_t)f * P1 + n] = v;
-      for (int dd = 0; dd < D; dd++) efn_w[((size_t)f * D + dd) * P1 + n] = d->opp_extrapolate_fn[((size_t)dd * F + f) * S + s];
+      efn_w[(size_t)f * P1 + n] = w;
       n++;
     }
     for (int m = n; m < P1; m++) ext_idx[(size_t)f * P1 + m] = ext_idx[(size_t)f * P1];
   }
   for (int dd = 0; dd < D; dd++)
-    for (int s = 0; s < S; s++) {
+    for (int q0 = 0; q0 < S; q0++) {
       int n = 0;
       for (int q = 0; q < S; q++) {
-        double v = d->opp_grad_spts[((size_t)dd * S + s) * S + q];
+        double v = d->opp_grad_spts[((size_t)dd * S + q0) * S + q];
         if (v == 0.) continue;
         if (n >= P1) return false;
-        grd_idx[((size_t)dd * S + s) * P1 + n] = q;
-        grd_w[((size_t)dd * S + s) * P1 + n] = v;
+        grd_idx[((size_t)dd * S + q0) * P1 + n] = spt_lds(D, P1, q);
+        grd_w[((size_t)dd * S + q0) * P1 + n] = v;
         n++;
       }
-      for (int m = n; m < P1; m++) grd_idx[((size_t)dd * S + s) * P1 + m] = s;
+      for (int m = n; m < P1; m++) grd_idx[((size_t)dd * S + q0) * P1 + m] = spt_lds(D, P1, q0);
     }
-  for (int s = 0; s < S; s++) {
+  for (int q = 0; q < S; q++) {
     int n = 0;
     for (int f = 0; f < F; f++) {
-      double v = d->opp_correction[(size_t)s * F + f];
-      bool nz = v != 0.;
-      if (c.prm.viscous)
-        for (int dd = 0; dd < D; dd++) nz = nz || d->opp_correct_u[((size_t)dd * S + s) * F + f] != 0.;
-      if (!nz) continue;
+      double v = d->opp_correction[(size_t)q * F + f];
+      if (v == 0.) continue;
       if (n >= NC) return false;
-      cor_idx[(size_t)s * NC + n] = f;
-      cor_w[(size_t)s * NC + n] = v;
-      if (c.prm.viscous)
-        for (int dd = 0; dd < D; dd++) cu_w[((size_t)dd * S + s) * NC + n] = d->opp_correct_u[((size_t)dd * S + s) * F + f];
+      cor_idx[(size_t)q * NC + n] = f;
+      cor_w[(size_t)q * NC + n] = v;
       n++;
     }
+    if (c.prm.viscous)
+      for (int dd = 0; dd < D; dd++) {
+        int m = 0;
+        for (int f = 0; f < F; f++) {
+          double v = d->opp_correct_u[((size_t)dd * S + q) * F + f];
+          if (v == 0.) continue;
+          if (m >= 2) return false;
+          cu_idx[((size_t)dd * S + q) * 2 + m] = f;
+          cu_w[((size_t)dd * S + q) * 2 + m] = v;
+          m++;
+        }
+      }
   }
   auto put = [&](const void* p, size_t bytes, size_t& o) {
     o = (blob.size() + 255) / 256 * 256;
@@ -735,12 +775,14 @@ static bool build_tables(const Ctx& c, const fhip_desc* d, std::vector<char>& bl
   blob.clear();
   put(ext_idx.data(), ext_idx.size() * 4, off[0]);
   put(ext_w.data(), ext_w.size() * 8, off[1]);
-  put(efn_w.data(), efn_w.size() * 8, off[2]);
-  put(grd_idx.data(), grd_idx.size() * 4, off[3]);
-  put(grd_w.data(), grd_w.size() * 8, off[4]);
-  put(cor_idx.data(), cor_idx.size() * 4, off[5]);
-  put(cor_w.data(), cor_w.size() * 8, off[6]);
-  put(cu_w.data(), cu_w.size() * 8, off[7]);
+  put(efn_dir.data(), efn_dir.size() * 4, off[2]);
+  put(efn_w.data(), efn_w.size() * 8, off[3]);
+  put(grd_idx.data(), grd_idx.size() * 4, off[4]);
+  put(grd_w.data(), grd_w.size() * 8, off[5]);
+  put(cor_idx.data(), cor_idx.size() * 4, off[6]);
+  put(cor_w.data(), cor_w.size() * 8, off[7]);
+  put(cu_idx.data(), cu_idx.size() * 4, off[8]);
+  put(cu_w.data(), cu_w.size() * 8, off[9]);
   return true;
 }
 
@@ -765,16 +807,17 @@ bool fused_setup(Ctx& c, const fhip_desc* d) {
   const Sizes& z = c.z;
   if (z.P < 0 || z.P > 5) return false;
   std::vector<char> blob;
-  size_t off[8];
+  size_t off[10];
   if (!build_tables(c, d, blob, off)) return false;
   auto* st = new FusedState();
   c.fused = st;
   st->tab_mem = dal<char>(blob.size());
   FCHK(hipMemcpy(st->tab_mem, blob.data(), blob.size(), hipMemcpyHostToDevice));
   char* base = (char*)st->tab_mem;
-  st->tb = Tabs{(const int*)(base + off[0]), (const double*)(base + off[1]), (const double*)(base + off[2]),
-                (const int*)(base + off[3]), (const double*)(base + off[4]), (const int*)(base + off[5]),
-                (const double*)(base + off[6]), (const double*)(base + off[7])};
+  st->tb = Tabs{(const int*)(base + off[0]),    (const double*)(base + off[1]), (const int*)(base + off[2]),
+                (const double*)(base + off[3]), (const int*)(base + off[4]),    (const double*)(base + off[5]),
+                (const int*)(base + off[6]),    (const double*)(base + off[7]), (const int*)(base + off[8]),
+                (const double*)(base + off[9])};
   const long long E = z.E, S = z.S, F = z.F, N = z.N, D = z.D;
   st->Ue = dal<double>(E * N * S);
   st->U0e = dal<double>(E * N * S);
