@@ -245,7 +245,7 @@ class BatchedSolver:
         return [per[gi][j] for gi, j in self.index]
 
     def solve_steady(self, tol: float = 1e-10, max_steps: int = 100000, check_every: int = 50,
-                     res_type: int = 2, field: int = 0, stop_on_nan: bool = True) -> dict:
+                     res_type: int = 2, field: int = 0, stop_on_nan: bool = True, progress=None) -> dict:
         """Step until every member's residual (field `field`) is <= tol; freeze
         converged members.  A member whose residual turns NaN raises, or with
         stop_on_nan=False is frozen and reported in "diverged".  Returns steps,
@@ -276,6 +276,8 @@ class BatchedSolver:
                         self.converged[i] = True
                         t_conv[i] = now
             stopped = self.converged | diverged
+            if progress is not None:
+                progress(done, int(self.converged.sum()), int(diverged.sum()))
             for gi, g in enumerate(self.groups):
                 act = np.array([0 if stopped[i] else 1 for i, (gg, _) in enumerate(self.index) if gg == gi],
                                dtype=np.int32)

@@ -42,11 +42,14 @@ class Request:
     residual: float | None = None
     dof: int = 0
     result: np.ndarray | None = None
+    status: str = ""               # "converged", "max_steps", "diverged" or "done" (unsteady)
 
 
 def build_case(r: Request):
     if r.kind == "bump2d":
-        return cases.bump2d(nx=r.size[0], ny=r.size[1], order=r.order, mach=r.mach, seed=1000 + r.rid)
+        # settings of FlurryPP's P-multigrid channel case: tuned steady RK, local CFL 0.9
+        return cases.bump2d(nx=r.size[0], ny=r.size[1], order=r.order, mach=r.mach, seed=1000 + r.rid,
+                            time_type=6, cfl=0.9)
     if r.kind == "vortex3d":
         return cases.vortex3d(n=r.size[0], order=r.order, mach=r.mach)
     raise ValueError(r.kind)
@@ -101,7 +104,7 @@ class Server:
                 waiting.pop(0)
                 s = Solver(setup.params, setup=setup)
                 s.initializeSolution(U)
-                mg = MultiGrid(s, low_order=0, smooth_steps=1) if (self.pmg and r.steady_tol is not None
+                mg = MultiGrid(s, low_order=0, smooth_steps=2) if (self.pmg and r.steady_tol is not None
                                                                     and setup.order > 0) else None
                 r.admitted, r.dof = now(), setup.nEles * setup.nSpts
                 ctx[r.rid] = (s, need, setup.params.nRKSteps, mg)
@@ -140,12 +143,20 @@ class Server:
                         res = s.residual_norm(2)
                         r.residual = float(res[0])
                         if not np.isfinite(res).all():
-                            raise RuntimeError(f"request {r.rid}: non-finite residual")
-                        fin = r.residual <= r.steady_tol or r.steps >= r.max_steps
+                            fin, r.status = True, "diverged"
+                        elif r.residual <= r.steady_tol:
+                            fin, r.status = True, "converged"
+                        elif r.steps >= r.max_steps:
+                            fin, r.status = True, "max_steps"
                 else:
                     fin = r.steps >= r.n_steps
+                    if fin:
+                        r.status = "done"
                 if fin:
                     r.finished = now()
+                    if self.log is not None:
+                        self.log(f"request {r.rid} ({r.kind} P{r.order}, {r.precision}) {r.status} after "
+                                 f"{r.steps} steps, {r.finished - r.arrival:.3f} s")
                     if self.keep_results:
                         r.result = s.get_U()
                     if mg is not None:
@@ -158,9 +169,11 @@ class Server:
                     still.append(r)
             self.active = still
         wall = now()
-        lat = np.array([r.finished - r.arrival for r in self.done])
+        # time to solution of the requests that reached one (converged or finished their steps)
+        lat = np.array([r.finished - r.arrival for r in self.done if r.status in ("converged", "done")])
+        count = {k: sum(1 for r in self.done if r.status == k) for k in ("converged", "max_steps", "diverged", "done")}
         return {
-            "requests": len(self.done),
+            "requests": len(self.done), "status": count,
             "wall_s": wall,
             "p50_s": float(np.percentile(lat, 50)) if len(lat) else None,
             "p99_s": float(np.percentile(lat, 99)) if len(lat) else None,
@@ -171,8 +184,9 @@ class Server:
 
 def poisson_mix(n: int, rate: float, seed: int = 1, frac_2d: float = 0.7, steady_tol: float = 1e-6,
                 unsteady_steps: int = 100, n3d=(8, 16), p3d=(2, 4), nx2d=(16, 48), ny2d=(8, 16),
-                p2d=(1, 3), mixed_frac: float = 0.5) -> list[Request]:
-    """Poisson arrivals at `rate` per second of a 2-D steady / 3-D unsteady mix."""
+                p2d=(1, 3), mixed_frac: float = 0.5, max_steps: int = 3000) -> list[Request]:
+    """Poisson arrivals at `rate` per second of a 2-D steady / 3-D unsteady mix;
+    steady requests give up after max_steps."""
     rng = np.random.default_rng(seed)
     t = 0.0
     out = []
@@ -183,7 +197,7 @@ def poisson_mix(n: int, rate: float, seed: int = 1, frac_2d: float = 0.7, steady
             out.append(Request(i, "bump2d", (int(rng.integers(nx2d[0], nx2d[1] + 1)),
                                              int(rng.integers(ny2d[0], ny2d[1] + 1))),
                                int(rng.integers(p2d[0], p2d[1] + 1)), t, mach=float(rng.uniform(0.2, 0.5)),
-                               precision=prec, steady_tol=steady_tol))
+                               precision=prec, steady_tol=steady_tol, max_steps=max_steps))
         else:
             out.append(Request(i, "vortex3d", (int(rng.integers(n3d[0], n3d[1] + 1)),),
                                int(rng.integers(p3d[0], p3d[1] + 1)), t, mach=0.5, precision=prec,
