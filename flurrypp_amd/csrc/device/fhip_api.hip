@@ -418,6 +418,10 @@ static bool resolve_path(fhip::Ctx& c, int path, bool& mixed) {
   return fused;
 }
 
+// Set while a P-multigrid cycle is captured into one graph: the steps inside
+// it are captured kernel by kernel rather than as graph launches.
+static thread_local bool t_capturing_cycle = false;
+
 // One iteration of the main loop body: iter++, solver::update(src).  The
 // coarse P-multigrid levels run update(true) without advancing their counter.
 static void update_step(fhip::Ctx& c, bool fused, bool mixed, bool src = false, bool count = true) {
@@ -427,7 +431,7 @@ static void update_step(fhip::Ctx& c, bool fused, bool mixed, bool src = false, 
   if (fused) {
     // Steady-state steps replay one captured graph; the wave-speed refresh
     // step and timed runs launch kernel by kernel.
-    if (!refresh && !c.timing && c.use_graphs) {
+    if (!refresh && !c.timing && c.use_graphs && !t_capturing_cycle) {
       hipGraphExec_t& ge = c.graph[src ? 1 : 0][mixed ? 1 : 0];
       if (!ge) {
         hipGraph_t g = nullptr;
@@ -502,11 +506,19 @@ struct fhip_pmg {
   std::vector<double*> R, Pro, sol, corr;  // device; R[i]: i+1 -> i, Pro[i]: i -> i+1
   int smooth = 1;
   hipEvent_t ev = nullptr;
+  // one captured cycle per path, replayed on the fine level's stream; valid
+  // while no level has re-allocated arrays since (graph_gen of every level)
+  hipGraphExec_t graph[4] = {nullptr, nullptr, nullptr, nullptr};
+  std::vector<unsigned long long> gen[4];
+  bool warm[4] = {false, false, false, false};
 };
 
 static void pmg_free(fhip_pmg* mg) {
   if (!mg) return;
   if (!mg->lv.empty()) (void)hipSetDevice(mg->lv.back()->c.device);
+  if (!mg->lv.empty()) (void)hipStreamSynchronize(mg->lv.back()->c.stream);
+  for (hipGraphExec_t g : mg->graph)
+    if (g) (void)hipGraphExecDestroy(g);
   for (auto* v : {&mg->R, &mg->Pro, &mg->sol, &mg->corr})
     for (double* p : *v)
       if (p) (void)hipFree(p);
@@ -587,19 +599,12 @@ static void pmg_restrict(fhip_pmg* mg, int i) {
   fhip::op_apply(c, "pmg_restrict", c.z.S, f.z.S, mg->R[i], f.divF, c.divF, false);
 }
 
-// multiGrid::cycle (src/multigrid.cpp:169-313), P levels.
-int fhip_pmg_cycle(fhip_pmg* mg, int path) {
-  return guard([&] {
-    require(mg, "null multigrid");
-    const int n = (int)mg->lv.size();
-    std::vector<char> fused(n), mixed(n);
-    for (int i = 0; i < n; i++) {
-      bool m;
-      fused[i] = resolve_path(mg->lv[i]->c, path, m);
-      mixed[i] = m;
-    }
-    fhip::Ctx& fine = mg->lv[n - 1]->c;
-    HCHK(hipSetDevice(fine.device));
+// multiGrid::cycle (src/multigrid.cpp:169-313), P levels: the launches of one
+// cycle, on the levels' streams (joined back into the fine level's stream).
+static void pmg_cycle_body(fhip_pmg* mg, const std::vector<char>& fused, const std::vector<char>& mixed) {
+  const int n = (int)mg->lv.size();
+  fhip::Ctx& fine = mg->lv[n - 1]->c;
+  {
     pmg_residual(fine, fused[n - 1], mixed[n - 1]);
     pmg_restrict(mg, n - 2);
     for (int i = n - 2; i >= 0; i--) {
@@ -626,6 +631,63 @@ int fhip_pmg_cycle(fhip_pmg* mg, int path) {
       pmg_handoff(mg, g, up);
       fhip::op_apply(up, "pmg_prolong", up.z.S, g.z.S, mg->Pro[i], mg->corr[i], up.U, true);
     }
+  }
+}
+
+// The first cycle on a path runs launch by launch (it also allocates what the
+// fused paths allocate on first use); the second is captured into one graph
+// across all levels' streams, which later cycles replay on the fine level's
+// stream.  Timed levels, disabled graphs or re-allocated arrays fall back to
+// launch-by-launch cycles (and a new capture).
+int fhip_pmg_cycle(fhip_pmg* mg, int path) {
+  return guard([&] {
+    require(mg, "null multigrid");
+    const int n = (int)mg->lv.size();
+    std::vector<char> fused(n), mixed(n);
+    bool graphs = true;
+    std::vector<unsigned long long> gen(n);
+    for (int i = 0; i < n; i++) {
+      bool m;
+      fhip::Ctx& c = mg->lv[i]->c;
+      fused[i] = resolve_path(c, path, m);
+      mixed[i] = m;
+      graphs = graphs && c.use_graphs && !c.timing;
+      gen[i] = c.graph_gen;
+    }
+    fhip::Ctx& fine = mg->lv[n - 1]->c;
+    HCHK(hipSetDevice(fine.device));
+    hipGraphExec_t& ge = mg->graph[path & 3];
+    if (ge && (!graphs || mg->gen[path & 3] != gen)) {
+      HCHK(hipStreamSynchronize(fine.stream));
+      (void)hipGraphExecDestroy(ge);
+      ge = nullptr;
+    }
+    if (!graphs || !mg->warm[path & 3]) {
+      pmg_cycle_body(mg, fused, mixed);
+      mg->warm[path & 3] = true;
+      return 0;
+    }
+    if (!ge) {
+      hipGraph_t g = nullptr;
+      HCHK(hipStreamBeginCapture(fine.stream, hipStreamCaptureModeThreadLocal));
+      t_capturing_cycle = true;
+      try {
+        pmg_cycle_body(mg, fused, mixed);
+      } catch (...) {
+        t_capturing_cycle = false;
+        (void)hipStreamEndCapture(fine.stream, &g);
+        if (g) (void)hipGraphDestroy(g);
+        throw;
+      }
+      t_capturing_cycle = false;
+      HCHK(hipStreamEndCapture(fine.stream, &g));
+      hipError_t e = hipGraphInstantiate(&ge, g, nullptr, nullptr, 0);
+      (void)hipGraphDestroy(g);
+      HCHK(e);
+      for (int i = 0; i < n; i++) gen[i] = mg->lv[i]->c.graph_gen;
+      mg->gen[path & 3] = gen;
+    }
+    HCHK(hipGraphLaunch(ge, fine.stream));
     return 0;
   });
 }

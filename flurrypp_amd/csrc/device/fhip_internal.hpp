@@ -88,7 +88,9 @@ struct Ctx {
   // A graph holds the device pointers of the moment it was captured: drop
   // them whenever an array they use is (re)allocated.
   hipGraphExec_t graph[2][3] = {{nullptr, nullptr, nullptr}, {nullptr, nullptr, nullptr}};
+  unsigned long long graph_gen = 0;  // bumped by drop_graphs: graphs over several contexts check it
   void drop_graphs() {
+    graph_gen++;
     bool any = false;
     for (auto& row : graph)
       for (auto& g : row) any = any || g != nullptr;

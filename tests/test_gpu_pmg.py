@@ -62,3 +62,29 @@ def test_pmg_cycle_mixed_precision(name):
     errs = _run(name, PATH_MIXED)
     print(f"mixed pmg {name}: {errs}")
     assert max(errs) <= 1e-6, errs
+
+
+@pytest.mark.parametrize("path", [PATH_FUSED, PATH_MIXED, PATH_STAGED])
+def test_pmg_cycle_graph_replay_matches_eager(path):
+    """From the third cycle on, fhip_pmg_cycle replays one graph captured over
+    all levels' streams; every replay equals the launch-by-launch cycle bit for
+    bit, with eager work and synchronisation between the replays."""
+    from flurrypp_amd import _native as nat
+    runs = []
+    for graphs in (1, 0):
+        setup, U = cases.bump2d(nx=10, ny=5, order=3, mach=0.35, time_type=6, cfl=0.9)
+        fine = Solver(setup.params, setup=setup)
+        fine.initializeSolution(U)
+        mg = MultiGrid(fine, low_order=0 if path != PATH_FUSED else 1, smooth_steps=2)
+        for s in mg.levels:
+            nat.hip_check(nat.hip_lib().fhip_set_option(s._ctx, 1, graphs))
+        states = []
+        for _ in range(6):
+            fine.update(1, path=path)
+            mg.cycle(path)
+            states.append((fine.get_U(), fine.residual_norm(2), mg.coarse[0].get_U()))
+        runs.append(states)
+        mg.close()
+        fine.close()
+    for i, (a, b) in enumerate(zip(*runs)):
+        assert np.array_equal(a[0], b[0]) and np.array_equal(a[1], b[1]) and np.array_equal(a[2], b[2]), f"cycle {i}"
