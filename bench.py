@@ -70,8 +70,9 @@ def batched_form(n_cases: int, seed: int, tol: float, max_steps: int, nx_hi: int
         s, U = cases.bump2d(nx=nx, ny=ny, order=P, mach=M, seed=seed * 100003 + i, time_type=6, cfl=0.9)
         ms.append(Member(s, U, (nx, ny, P, M)))
     t = time.perf_counter()
-    b = BatchedSolver(ms, pmg=True, low_order=0, smooth_steps=2)
+    b = BatchedSolver(ms, pmg=True, low_order=0, smooth_steps=2, autotune=True)
     setup_s = time.perf_counter() - t
+    tuned = {f"P{g}/level P{q}": v for (g, q), v in sorted(b.block_sizes.items())}
     def progress(step, nconv, ndiv):
         if step % 500 == 0:
             log(f"bench: batched step {step}: {nconv} converged, {ndiv} diverged")
@@ -94,7 +95,7 @@ def batched_form(n_cases: int, seed: int, tol: float, max_steps: int, nx_hi: int
         "p99_s": float(np.percentile(tc, 99)) if len(tc) else None,
         "steps_p50": int(np.percentile(out["steps"], 50)), "steps_max": int(out["steps"].max()),
         "value": round(out["dof_stages"] / out["wall_s"] / 1e9, 4), "unit": "GDOF·stage/s (fine-level RK stages)",
-        "host_setup_s": round(setup_s, 2),
+        "host_setup_s": round(setup_s, 2), "block_sizes": tuned,
     }
 
 
@@ -147,6 +148,7 @@ def main() -> int:
     ap.add_argument("--cpu-steps", type=int, default=8)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-mixed", action="store_true")
+    ap.add_argument("--no-tune", action="store_true", help="keep the default block sizes (256)")
     ap.add_argument("--batch-cases", type=int, default=64, help="0 skips the batched form")
     ap.add_argument("--batch-tol", type=float, default=1e-10)
     ap.add_argument("--batch-max-steps", type=int, default=3000)
@@ -173,6 +175,14 @@ def main() -> int:
     s = Solver(setup.params, setup=setup)
     s.initializeSolution(U)
     path = PATH_FUSED if a.path == "fused" else PATH_STAGED
+    tuned = {}
+    if a.path == "fused" and not a.no_tune:
+        from flurrypp_amd._native import PATH_MIXED
+        t = time.perf_counter()
+        tuned["fp64"] = s.autotune(PATH_FUSED)
+        if not a.no_mixed:
+            tuned["mixed"] = s.autotune(PATH_MIXED)
+        log(f"bench: autotuned block sizes {tuned} in {time.perf_counter() - t:.1f} s")
     s.update(a.warmup, path=path)
     s.check_state()
     log("bench: warmup done")
@@ -276,7 +286,8 @@ def main() -> int:
         "config": {"workload": f"3D {'Navier-Stokes' if visc else 'Euler'} Taylor-Green vortex, {a.n}^3 periodic "
                                f"hex, P{a.order}, Rusanov{'+LDG' if visc else ''}, RK44, FP64",
                    "elements": setup.nEles, "dof": dof, "stages_per_step": nRK, "path": a.path,
-                   "dt": setup.params.dt},
+                   "dt": setup.params.dt,
+                   "block_sizes": tuned or "default 256 (extrapolate, face_inviscid, volume, face_viscous, rk)"},
         "roofline": roof,
         "cpu_baseline": cpu,
         "kernels": kernels,

@@ -109,6 +109,9 @@ def hip_lib() -> C.CDLL:
         lib.fhip_mesh_residual_norms.argtypes = [C.c_void_p, C.c_int, _dp]
         lib.fhip_set_mesh_active.argtypes = [C.c_void_p, _ip]
         lib.fhip_update_many.argtypes = [C.POINTER(C.c_void_p), C.c_int, C.c_int, C.c_int]
+        for name in ("fhip_autotune", "fhip_get_block_sizes"):
+            getattr(lib, name).argtypes = [C.c_void_p, C.c_int, _ip]
+        lib.fhip_set_block_sizes.argtypes = [C.c_void_p, C.c_int, _ip]
         lib.fhip_pmg_create.argtypes = [C.POINTER(C.c_void_p), C.c_int, C.POINTER(_dp), C.POINTER(_dp), C.c_int,
                                         C.POINTER(C.c_void_p)]
         lib.fhip_pmg_cycle.argtypes = [C.c_void_p, C.c_int]

@@ -154,7 +154,7 @@ class BatchedSolver:
     of its updates."""
 
     def __init__(self, members: list[Member], path: int = nat.PATH_FUSED, pmg: bool = False, low_order: int = 0,
-                 smooth_steps: int = 1):
+                 smooth_steps: int = 1, autotune: bool = False):
         self.members = members
         self.path = path
         self.pmg = pmg
@@ -210,6 +210,14 @@ class BatchedSolver:
                 h = C.c_void_p()
                 nat.hip_check(nat.hip_lib().fhip_pmg_create(hs_, n, Ra, Pa, smooth_steps, C.byref(h)))
                 self._mg[gi] = h
+        # block sizes of every context's kernels, timed on its own data
+        self.block_sizes = {}
+        if autotune and path in (nat.PATH_FUSED, nat.PATH_MIXED):
+            for gi, g in enumerate(self.groups):
+                for lv in self.coarse[gi] + [g]:
+                    out = np.zeros(5, dtype=np.int32)
+                    nat.hip_check(nat.hip_lib().fhip_autotune(lv._ctx, path, nat.iptr(out)))
+                    self.block_sizes[(g.m.order, lv.m.order)] = out.tolist()
 
     def close(self):
         for h in self._mg:

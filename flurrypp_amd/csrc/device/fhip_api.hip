@@ -718,6 +718,43 @@ int fhip_path_supported(fhip_ctx* h, int path) {
   return 0;
 }
 
+static bool tuning_path(fhip::Ctx& c, int path) {
+  bool mixed;
+  const bool fused = resolve_path(c, path, mixed);
+  require(fused, "block sizes apply to the fused paths");
+  return mixed;
+}
+
+int fhip_autotune(fhip_ctx* h, int path, int* block_sizes) {
+  return guard([&] {
+    require(h, "null context");
+    fhip::Ctx& c = h->c;
+    const bool mixed = tuning_path(c, path);
+    fhip::fused_autotune(c, mixed, block_sizes);
+    return 0;
+  });
+}
+
+int fhip_set_block_sizes(fhip_ctx* h, int path, const int* block_sizes) {
+  return guard([&] {
+    require(h && block_sizes, "null argument");
+    fhip::Ctx& c = h->c;
+    const bool mixed = tuning_path(c, path);
+    fhip::fused_set_blocks(c, mixed, block_sizes);
+    return 0;
+  });
+}
+
+int fhip_get_block_sizes(fhip_ctx* h, int path, int* block_sizes) {
+  return guard([&] {
+    require(h && block_sizes, "null argument");
+    fhip::Ctx& c = h->c;
+    const bool mixed = tuning_path(c, path);
+    fhip::fused_get_blocks(c, mixed, block_sizes);
+    return 0;
+  });
+}
+
 int fhip_set_option(fhip_ctx* h, int option, int value) {
   return guard([&] {
     require(h, "null context");

@@ -26,6 +26,9 @@
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
+#include <array>
+#include <map>
+#include <mutex>
 #include <cmath>
 #include <cstdio>
 #include <cstring>
@@ -51,10 +54,16 @@ struct Shape {
   static constexpr int S = (D == 2) ? P1 * P1 : P1 * P1 * P1;
   static constexpr int F = 2 * D * Ff;
   static constexpr int N = D + 2;
-  static constexpr int NC = 2 * D;                          // correction entries per solution point
-  static constexpr int EB = (256 / S) > 0 ? (256 / S) : 1;  // elements per block
-  static constexpr int NT = 256;
+  static constexpr int NC = 2 * D;  // correction entries per solution point
 };
+
+// Element kernels take a runtime block of NT >= S threads holding EB = NT/S
+// elements, with their LDS slabs in dynamic shared memory; the block size of
+// each kernel is chosen per context (default 256, or by fused_autotune).
+extern __shared__ __align__(16) unsigned char fhip_lds[];
+constexpr int kMaxBlock = 512;
+constexpr size_t kMaxLds = 64 * 1024;
+enum { kExtrap = 0, kFaceInv = 1, kVolume = 2, kFaceVisc = 3, kRk = 4 };
 
 // Sparse operator tables (device pointers into one allocation).  Indices of
 // solution points are positions in the swizzled LDS layout (spt_lds below).
@@ -114,6 +123,8 @@ struct FusedState {
   // P-multigrid source term (element-major copy of src_spts), used while use_src
   double* srce = nullptr;
   bool use_src = false;
+  // block size of each kernel (kExtrap..kRk) for FP64 [0] and mixed [1]
+  int nt[2][5] = {{256, 256, 256, 256, 256}, {256, 256, 256, 256, 256}};
   Work<double> w64;
   Work<float> w32;
   ~FusedState() {
@@ -158,10 +169,12 @@ __global__ void k_from_em(long long total, int P, int E, int K, const double* __
 // ---------------------------------------------------------------------------
 // U at the flux points from U at the solution points (opp_spts_to_fpts rows).
 template <int D, int P, class R>
-__global__ __launch_bounds__(256) void k_extrap(int E, const double* __restrict__ Ue, R* __restrict__ Ufe, Tabs tb) {
+__global__ __launch_bounds__(kMaxBlock) void k_extrap(int E, const double* __restrict__ Ue, R* __restrict__ Ufe,
+                                                       Tabs tb) {
   using Sh = Shape<D, P>;
-  constexpr int S = Sh::S, F = Sh::F, N = Sh::N, EB = Sh::EB, P1 = Sh::P1;
-  __shared__ R sU[EB * N * S];
+  constexpr int S = Sh::S, F = Sh::F, N = Sh::N, P1 = Sh::P1;
+  const int EB = blockDim.x / S;
+  R* const sU = reinterpret_cast<R*>(fhip_lds);
   const int e0 = blockIdx.x * EB, nb = min(EB, E - e0);
   const double* src = Ue + (long long)e0 * N * S;
   for (int i = threadIdx.x; i < nb * N * S; i += blockDim.x) {
@@ -203,7 +216,7 @@ __device__ __forceinline__ void bc_state(int code, const R* UL, const R* n, R* U
 }
 
 template <int D, int P, class R>
-__global__ __launch_bounds__(256) void k_face_inv(int nFF, const int* __restrict__ pLa, const int* __restrict__ pRa,
+__global__ __launch_bounds__(kMaxBlock) void k_face_inv(int nFF, const int* __restrict__ pLa, const int* __restrict__ pRa,
                                                    const int* __restrict__ bca, const R* __restrict__ Ufe,
                                                    const double* __restrict__ norme, const double* __restrict__ dAe,
                                                    R* __restrict__ Fne, R* __restrict__ FnInv,
@@ -261,18 +274,18 @@ __global__ __launch_bounds__(256) void k_face_inv(int nFF, const int* __restrict
 // [and gradient at faces].  One thread per solution point, EB elements per
 // block, element data staged in LDS.
 template <int D, int P, bool VISC, class R>
-__global__ __launch_bounds__(256) void k_volume(int E, const double* __restrict__ Ue, const R* __restrict__ dUce,
+__global__ __launch_bounds__(kMaxBlock) void k_volume(int E, const double* __restrict__ Ue, const R* __restrict__ dUce,
                                                  const double* __restrict__ JGe, const double* __restrict__ detJe,
                                                  R* __restrict__ divFe, R* __restrict__ disFne, R* __restrict__ dUfe,
                                                  Tabs tb, Phys ph) {
   using Sh = Shape<D, P>;
-  constexpr int S = Sh::S, F = Sh::F, N = Sh::N, EB = Sh::EB, P1 = Sh::P1, NC = Sh::NC;
+  constexpr int S = Sh::S, F = Sh::F, N = Sh::N, P1 = Sh::P1, NC = Sh::NC;
+  const int EB = blockDim.x / S;
   // sF holds, in turn (viscous runs): dUc at the flux points, the physical
   // gradient until it has been extrapolated to the faces, then the transformed
-  // flux at the same indices as the gradient.
-  constexpr int SF = (VISC && F > D * S) ? F : D * S;
-  __shared__ R sU[EB * N * S];
-  __shared__ R sF[EB * N * SF];
+  // flux at the same indices as the gradient.  [EB*N*S] + [EB*N*SF]
+  R* const sU = reinterpret_cast<R*>(fhip_lds);
+  R* const sF = sU + EB * N * S;
   R* const sG = sF;
   const int e0 = blockIdx.x * EB, nb = min(EB, E - e0);
   {
@@ -431,7 +444,7 @@ __global__ __launch_bounds__(256) void k_volume(int E, const double* __restrict_
 // ---------------------------------------------------------------------------
 // LDG common viscous flux at face flux points.
 template <int D, int P, class R>
-__global__ __launch_bounds__(256) void k_face_visc(int nFF, const int* __restrict__ pLa, const int* __restrict__ pRa,
+__global__ __launch_bounds__(kMaxBlock) void k_face_visc(int nFF, const int* __restrict__ pLa, const int* __restrict__ pRa,
                                                     const int* __restrict__ bca, const R* __restrict__ Ufe,
                                                     const R* __restrict__ dUfe, const double* __restrict__ norme,
                                                     const double* __restrict__ dAe, R* __restrict__ Fne,
@@ -492,7 +505,7 @@ __global__ __launch_bounds__(256) void k_face_visc(int nFF, const int* __restric
 // next stage.  mode: 0 first classical stage, 1 middle classical stage,
 // 2 last classical stage, 3 single-stage forward Euler, 4 Jameson-form stage.
 template <int D, int P, class R>
-__global__ __launch_bounds__(256) void k_rk(int E, int mode, double a, double bcoef, const R* __restrict__ divFe,
+__global__ __launch_bounds__(kMaxBlock) void k_rk(int E, int mode, double a, double bcoef, const R* __restrict__ divFe,
                                              const R* __restrict__ Fne, const R* __restrict__ disFne,
                                              const double* __restrict__ detJe, const double* __restrict__ dtp,
                                              const double* __restrict__ eleDt, int local, double* __restrict__ Ue,
@@ -501,9 +514,10 @@ __global__ __launch_bounds__(256) void k_rk(int E, int mode, double a, double bc
                                              const int* __restrict__ eleMesh, const double* __restrict__ meshDt,
                                              const int* __restrict__ active, const double* __restrict__ srce) {
   using Sh = Shape<D, P>;
-  constexpr int S = Sh::S, F = Sh::F, N = Sh::N, EB = Sh::EB, P1 = Sh::P1, NC = Sh::NC;
-  __shared__ R sU[EB * N * S];
-  __shared__ R sD[EB * N * F];  // Fn - disFn at the block's flux points
+  constexpr int S = Sh::S, F = Sh::F, N = Sh::N, P1 = Sh::P1, NC = Sh::NC;
+  const int EB = blockDim.x / S;
+  R* const sU = reinterpret_cast<R*>(fhip_lds);  // [EB*N*S]
+  R* const sD = sU + EB * N * S;                  // [EB*N*F] Fn - disFn at the block's flux points
   const int e0 = blockIdx.x * EB, nb = min(EB, E - e0);
   const int b = threadIdx.x / S, s = threadIdx.x - b * S;
   {
@@ -945,13 +959,73 @@ void fused_release(Ctx& c) {
 template <int D, int P, class R>
 struct Runner {
   using Sh = Shape<D, P>;
+  static constexpr int S = Sh::S, F = Sh::F, N = Sh::N;
   static const char* nm(const char* f64, const char* f32) { return sizeof(R) == 8 ? f64 : f32; }
-  static void extrap(Ctx& c) {
+  static int& blk(Ctx& c, int k) { return fs(c).nt[sizeof(R) == 4 ? 1 : 0][k]; }
+  // LDS bytes of an element kernel at block size nt (0 for the face kernels)
+  static size_t lds(int k, int nt, bool visc) {
+    const size_t EB = nt / S;
+    const size_t SF = (visc && F > D * S) ? F : D * S;
+    if (k == kExtrap) return EB * N * S * sizeof(R);
+    if (k == kVolume) return EB * N * (S + SF) * sizeof(R);
+    if (k == kRk) return EB * N * (S + F) * sizeof(R);
+    return 0;
+  }
+  static bool valid(int k, int nt, bool visc) {
+    if (nt < 64 || nt > kMaxBlock) return false;
+    if (k == kFaceInv || k == kFaceVisc) return true;
+    return nt >= S && lds(k, nt, visc) <= kMaxLds;
+  }
+  static void launch_extrap(Ctx& c, int nt) {
     FusedState& st = fs(c);
     Work<R>& w = st.work<R>();
-    Launch L(c, nm("fused_extrapolate", "mixed_extrapolate"));
-    hipLaunchKernelGGL((k_extrap<D, P, R>), dim3(nblk(c.z.E, Sh::EB)), dim3(Sh::NT), 0, c.stream, c.z.E, st.Ue, w.Ufe, st.tb);
+    hipLaunchKernelGGL((k_extrap<D, P, R>), dim3(nblk(c.z.E, nt / S)), dim3(nt), lds(kExtrap, nt, false), c.stream,
+                       c.z.E, st.Ue, w.Ufe, st.tb);
     FCHK(hipGetLastError());
+  }
+  static void launch_face_inv(Ctx& c, int nt) {
+    FusedState& st = fs(c);
+    Work<R>& w = st.work<R>();
+    const int nFF = c.z.nFaceFpts;
+    if (nFF == 0) return;
+    hipLaunchKernelGGL((k_face_inv<D, P, R>), dim3(nblk(nFF, nt)), dim3(nt), 0, c.stream, nFF, st.pL, st.pR, st.bc,
+                       w.Ufe, st.norme, st.dAe, w.Fne, w.FnInv, st.waveSpe, w.dUce, c.ph, st.bst);
+    FCHK(hipGetLastError());
+  }
+  static void launch_volume(Ctx& c, int nt) {
+    FusedState& st = fs(c);
+    Work<R>& w = st.work<R>();
+    const bool visc = c.prm.viscous != 0;
+    if (visc)
+      hipLaunchKernelGGL((k_volume<D, P, true, R>), dim3(nblk(c.z.E, nt / S)), dim3(nt), lds(kVolume, nt, true),
+                         c.stream, c.z.E, st.Ue, w.dUce, st.JGe, st.detJe, w.divFe, w.disFne, w.dUfe, st.tb, c.ph);
+    else
+      hipLaunchKernelGGL((k_volume<D, P, false, R>), dim3(nblk(c.z.E, nt / S)), dim3(nt), lds(kVolume, nt, false),
+                         c.stream, c.z.E, st.Ue, w.dUce, st.JGe, st.detJe, w.divFe, w.disFne, w.dUfe, st.tb, c.ph);
+    FCHK(hipGetLastError());
+  }
+  static void launch_face_visc(Ctx& c, int nt) {
+    FusedState& st = fs(c);
+    Work<R>& w = st.work<R>();
+    const int nFF = c.z.nFaceFpts;
+    if (nFF == 0) return;
+    hipLaunchKernelGGL((k_face_visc<D, P, R>), dim3(nblk(nFF, nt)), dim3(nt), 0, c.stream, nFF, st.pL, st.pR, st.bc,
+                       w.Ufe, w.dUfe, st.norme, st.dAe, w.Fne, w.FnInv, c.ph, st.bst);
+    FCHK(hipGetLastError());
+  }
+  static void launch_rk(Ctx& c, int nt, int mode, double a, double b, bool write0) {
+    FusedState& st = fs(c);
+    Work<R>& w = st.work<R>();
+    hipLaunchKernelGGL((k_rk<D, P, R>), dim3(nblk(c.z.E, nt / S)), dim3(nt), lds(kRk, nt, false), c.stream, c.z.E, mode,
+                       a, b, w.divFe, w.Fne, w.disFne, st.detJe, c.d_dt, c.eleDt,
+                       c.prm.dt_type == 2 ? 1 : (c.prm.dt_type == 1 && st.nMesh > 1 ? 2 : 0), st.Ue, st.U0e, st.Uacc,
+                       write0 ? st.divF0e : nullptr, w.Ufe, st.tb, st.eleMesh, st.meshDt, st.active,
+                       st.use_src ? st.srce : nullptr);
+    FCHK(hipGetLastError());
+  }
+  static void extrap(Ctx& c) {
+    Launch L(c, nm("fused_extrapolate", "mixed_extrapolate"));
+    launch_extrap(c, blk(c, kExtrap));
   }
   static void wave(Ctx& c) {
     FusedState& st = fs(c);
@@ -978,44 +1052,87 @@ struct Runner {
     FCHK(hipGetLastError());
   }
   static void residual(Ctx& c) {
-    FusedState& st = fs(c);
-    Work<R>& w = st.work<R>();
-    const int nFF = c.z.nFaceFpts;
     const bool visc = c.prm.viscous != 0;
-    if (nFF > 0) {
+    {
       Launch L(c, nm("fused_face_inviscid", "mixed_face_inviscid"));
-      hipLaunchKernelGGL((k_face_inv<D, P, R>), dim3(nblk(nFF, 256)), dim3(256), 0, c.stream, nFF, st.pL, st.pR, st.bc,
-                         w.Ufe, st.norme, st.dAe, w.Fne, w.FnInv, st.waveSpe, w.dUce, c.ph, st.bst);
-      FCHK(hipGetLastError());
+      launch_face_inv(c, blk(c, kFaceInv));
     }
     {
       Launch L(c, visc ? nm("fused_volume_viscous", "mixed_volume_viscous")
                        : nm("fused_volume_inviscid", "mixed_volume_inviscid"));
-      if (visc)
-        hipLaunchKernelGGL((k_volume<D, P, true, R>), dim3(nblk(c.z.E, Sh::EB)), dim3(Sh::NT), 0, c.stream, c.z.E, st.Ue,
-                           w.dUce, st.JGe, st.detJe, w.divFe, w.disFne, w.dUfe, st.tb, c.ph);
-      else
-        hipLaunchKernelGGL((k_volume<D, P, false, R>), dim3(nblk(c.z.E, Sh::EB)), dim3(Sh::NT), 0, c.stream, c.z.E, st.Ue,
-                           w.dUce, st.JGe, st.detJe, w.divFe, w.disFne, w.dUfe, st.tb, c.ph);
-      FCHK(hipGetLastError());
+      launch_volume(c, blk(c, kVolume));
     }
-    if (visc && nFF > 0) {
+    if (visc) {
       Launch L(c, nm("fused_face_viscous", "mixed_face_viscous"));
-      hipLaunchKernelGGL((k_face_visc<D, P, R>), dim3(nblk(nFF, 256)), dim3(256), 0, c.stream, nFF, st.pL, st.pR, st.bc,
-                         w.Ufe, w.dUfe, st.norme, st.dAe, w.Fne, w.FnInv, c.ph, st.bst);
-      FCHK(hipGetLastError());
+      launch_face_visc(c, blk(c, kFaceVisc));
     }
   }
   static void rk(Ctx& c, int mode, double a, double b, bool write0) {
-    FusedState& st = fs(c);
-    Work<R>& w = st.work<R>();
     Launch L(c, nm("fused_rk_update", "mixed_rk_update"));
-    hipLaunchKernelGGL((k_rk<D, P, R>), dim3(nblk(c.z.E, Sh::EB)), dim3(Sh::NT), 0, c.stream, c.z.E, mode, a, b, w.divFe,
-                       w.Fne, w.disFne, st.detJe, c.d_dt, c.eleDt,
-                       c.prm.dt_type == 2 ? 1 : (c.prm.dt_type == 1 && st.nMesh > 1 ? 2 : 0), st.Ue, st.U0e, st.Uacc,
-                       write0 ? st.divF0e : nullptr, w.Ufe, st.tb, st.eleMesh, st.meshDt, st.active,
-                       st.use_src ? st.srce : nullptr);
-    FCHK(hipGetLastError());
+    launch_rk(c, blk(c, kRk), mode, a, b, write0);
+  }
+  // Time every valid block size of every kernel on this context's data and
+  // keep the fastest.  The RK kernel runs in its middle-stage mode, which
+  // rewrites U and the RK accumulator: both are saved and restored, as are the
+  // wave speeds the face kernel writes; block shapes never change results.
+  static void tune(Ctx& c) {
+    FusedState& st = fs(c);
+    const bool visc = c.prm.viscous != 0;
+    const long long nS = (long long)c.z.E * N * S, nW = (long long)c.z.E * F;
+    double *bU = nullptr, *bA = nullptr, *bW = nullptr;
+    FCHK(hipMalloc(&bU, nS * 8));
+    FCHK(hipMalloc(&bA, nS * 8));
+    FCHK(hipMalloc(&bW, nW * 8));
+    FCHK(hipMemcpyAsync(bU, st.Ue, nS * 8, hipMemcpyDeviceToDevice, c.stream));
+    FCHK(hipMemcpyAsync(bA, st.Uacc, nS * 8, hipMemcpyDeviceToDevice, c.stream));
+    FCHK(hipMemcpyAsync(bW, st.waveSpe, nW * 8, hipMemcpyDeviceToDevice, c.stream));
+    hipEvent_t e0, e1;
+    FCHK(hipEventCreate(&e0));
+    FCHK(hipEventCreate(&e1));
+    const int cand[4] = {64, 128, 256, 512};
+    for (int k = kExtrap; k <= kRk; k++) {
+      if (k == kFaceVisc && !visc) continue;
+      auto run = [&](int nt) {
+        switch (k) {
+          case kExtrap: launch_extrap(c, nt); break;
+          case kFaceInv: launch_face_inv(c, nt); break;
+          case kVolume: launch_volume(c, nt); break;
+          case kFaceVisc: launch_face_visc(c, nt); break;
+          default: launch_rk(c, nt, 1, 0.5, 1. / 3., false);
+        }
+      };
+      int best = blk(c, k);
+      float bt = 1e30f;
+      for (int nt : cand) {
+        if (!valid(k, nt, visc)) continue;
+        run(nt);
+        FCHK(hipEventRecord(e0, c.stream));
+        for (int r = 0; r < 3; r++) run(nt);
+        FCHK(hipEventRecord(e1, c.stream));
+        FCHK(hipEventSynchronize(e1));
+        float ms = 0;
+        FCHK(hipEventElapsedTime(&ms, e0, e1));
+        if (ms < bt) bt = ms, best = nt;
+      }
+      blk(c, k) = best;
+      run(best);  // leave this kernel's outputs for the next one's inputs
+    }
+    FCHK(hipMemcpyAsync(st.Ue, bU, nS * 8, hipMemcpyDeviceToDevice, c.stream));
+    FCHK(hipMemcpyAsync(st.Uacc, bA, nS * 8, hipMemcpyDeviceToDevice, c.stream));
+    FCHK(hipMemcpyAsync(st.waveSpe, bW, nW * 8, hipMemcpyDeviceToDevice, c.stream));
+    FCHK(hipStreamSynchronize(c.stream));
+    (void)hipEventDestroy(e0);
+    (void)hipEventDestroy(e1);
+    (void)hipFree(bU);
+    (void)hipFree(bA);
+    (void)hipFree(bW);
+    launch_extrap(c, blk(c, kExtrap));  // U at the faces of the restored state
+  }
+  static bool check(Ctx& c, const int* nt) {
+    const bool visc = c.prm.viscous != 0;
+    for (int k = kExtrap; k <= kRk; k++)
+      if (!valid(k, nt[k], visc)) return false;
+    return true;
   }
 };
 
@@ -1026,12 +1143,14 @@ struct Ops {
   void (*residual)(Ctx&);
   void (*rk)(Ctx&, int, double, double, bool);
   void (*ensure)(Ctx&);
+  void (*tune)(Ctx&);
+  bool (*check)(Ctx&, const int*);
 };
 
 template <int D, int P, class R>
 static Ops ops_for() {
   using Rn = Runner<D, P, R>;
-  return Ops{Rn::extrap, Rn::wave, Rn::dt, Rn::residual, Rn::rk, ensure_work<R>};
+  return Ops{Rn::extrap, Rn::wave, Rn::dt, Rn::residual, Rn::rk, ensure_work<R>, Rn::tune, Rn::check};
 }
 
 template <class R>
@@ -1058,6 +1177,52 @@ static Ops get_ops_r(const Ctx& c) {
 }
 
 static Ops get_ops(const Ctx& c, bool mixed) { return mixed ? get_ops_r<float>(c) : get_ops_r<double>(c); }
+
+// Tuned block sizes, shared by the contexts of one process: keyed by
+// (nDims, order, precision, viscous, log2 of the element count).
+static std::mutex g_tune_mu;
+static std::map<std::array<int, 5>, std::array<int, 5>> g_tuned;
+
+void fused_autotune(Ctx& c, bool mixed, int* out) {
+  FusedState& st = fs(c);
+  Ops o = get_ops(c, mixed);
+  const int m = mixed ? 1 : 0;
+  int lg = 0;
+  for (long long e = c.z.E; e > 1; e >>= 1) lg++;
+  const std::array<int, 5> key{c.z.D, c.z.P, m, c.prm.viscous ? 1 : 0, lg};
+  bool hit = false;
+  {
+    std::lock_guard<std::mutex> g(g_tune_mu);
+    auto it = g_tuned.find(key);
+    if (it != g_tuned.end()) {
+      std::copy(it->second.begin(), it->second.end(), st.nt[m]);
+      hit = true;
+    }
+  }
+  if (!hit) {
+    c.drop_graphs();
+    fused_begin(c, mixed, false);  // reads U and the wave speeds; both are left as they were
+    o.tune(c);
+    std::array<int, 5> v;
+    std::copy(st.nt[m], st.nt[m] + 5, v.begin());
+    std::lock_guard<std::mutex> g(g_tune_mu);
+    g_tuned[key] = v;
+  }
+  c.drop_graphs();
+  if (out) std::copy(st.nt[m], st.nt[m] + 5, out);
+}
+
+void fused_set_blocks(Ctx& c, bool mixed, const int* nt) {
+  Ops o = get_ops(c, mixed);
+  if (!o.check(c, nt)) throw std::invalid_argument("block size not valid for this kernel and order");
+  c.drop_graphs();
+  std::copy(nt, nt + 5, fs(c).nt[mixed ? 1 : 0]);
+}
+
+void fused_get_blocks(Ctx& c, bool mixed, int* out) {
+  const int* nt = fs(c).nt[mixed ? 1 : 0];
+  std::copy(nt, nt + 5, out);
+}
 
 static double cfl_limit_f(int order) {  // getCFLLimit, src/global.cpp:81-105
   static const double t[6] = {1.393, 0.464, 0.235, 0.1453, 0.100, 0.0736};

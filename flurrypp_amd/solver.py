@@ -278,6 +278,21 @@ class Solver:
         nat.hip_check(nat.hip_lib().fhip_check_state(self._ctx))
 
     # -- timing ------------------------------------------------------------------------
+    def autotune(self, path: int = nat.PATH_FUSED) -> list[int]:
+        """Pick the block size of every fused kernel by timing them on the current state."""
+        out = np.zeros(5, dtype=np.int32)
+        nat.hip_check(nat.hip_lib().fhip_autotune(self._ctx, path, nat.iptr(out)))
+        return out.tolist()
+
+    def block_sizes(self, path: int = nat.PATH_FUSED) -> list[int]:
+        out = np.zeros(5, dtype=np.int32)
+        nat.hip_check(nat.hip_lib().fhip_get_block_sizes(self._ctx, path, nat.iptr(out)))
+        return out.tolist()
+
+    def set_block_sizes(self, sizes, path: int = nat.PATH_FUSED) -> None:
+        a = np.ascontiguousarray(sizes, dtype=np.int32)
+        nat.hip_check(nat.hip_lib().fhip_set_block_sizes(self._ctx, path, nat.iptr(a)))
+
     def enable_timing(self, on: bool = True) -> None:
         nat.hip_lib().fhip_enable_timing(self._ctx, int(on))
 

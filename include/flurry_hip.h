@@ -188,6 +188,18 @@ int fhip_path_supported(fhip_ctx* ctx, int path);
 enum { FHIP_OPT_GRAPHS = 1 };
 int fhip_set_option(fhip_ctx* ctx, int option, int value);
 
+/* Block sizes (threads per block, 64..512) of the fused kernels, in the order
+ * extrapolate, face_inviscid, volume, face_viscous, rk_update, for path
+ * FHIP_PATH_FUSED or FHIP_PATH_MIXED (no counterpart in the reference, which
+ * has no kernels).  fhip_autotune times every valid size of every kernel on
+ * the context's current data, keeps the fastest and returns them in
+ * block_sizes (may be NULL); the state is left unchanged.  Results are cached
+ * for the process per (nDims, order, precision, viscous, log2 nEles).  Block
+ * shapes never change results.  Default 256 each. */
+int fhip_autotune(fhip_ctx* ctx, int path, int* block_sizes);
+int fhip_set_block_sizes(fhip_ctx* ctx, int path, const int* block_sizes);
+int fhip_get_block_sizes(fhip_ctx* ctx, int path, int* block_sizes);
+
 /* Residual norm over divF_spts[0] (resType 1, 2 or 3): out[nFields]. */
 int fhip_residual_norm(fhip_ctx* ctx, int res_type, double* out);
 
