@@ -216,6 +216,7 @@ class InputParams:
         p.errorNorm = g("errorNorm", int, 1)
         p.quadOrder = g("quadOrder", int, 10)
         p.resType = g("resType", int, 2)
+        p.plotType = g("plotType", int, 1)
         p.plotFreq = g("plotFreq", int, 100)
         p.dataFileName = g("dataFileName", str, "simData")
         p.sptsTypeQuad = g("spts_type_quad", str, "Legendre")

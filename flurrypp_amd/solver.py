@@ -11,6 +11,7 @@ from __future__ import annotations
 import copy
 import ctypes as C
 import math
+from pathlib import Path
 
 import numpy as np
 
@@ -205,8 +206,16 @@ class Solver:
         nat.hip_check(nat.hip_lib().fhip_set_array(self._ctx, nat.ARR[name], step, a.ctypes.data_as(C.c_void_p)))
 
     # -- reference members ---------------------------------------------------------
-    def initializeSolution(self, U: np.ndarray | None = None) -> None:
-        """solver::initializeSolution (src/solver.cpp:1892-1943); U overrides the IC."""
+    def initializeSolution(self, U: np.ndarray | None = None, directory: str = ".") -> None:
+        """solver::initializeSolution (src/solver.cpp:1892-1943); U overrides the
+        IC; with `restart 1` the state (and time) come from the plot file of
+        restartIter (solver::readRestartFile, src/solver.cpp:1789-1890)."""
+        p = self.params
+        if U is None and getattr(p, "restart", 0):
+            from .output import read_restart, vtu_name
+            U, t = read_restart(Path(directory) / vtu_name(p.dataFileName, p.restartIter), self.host)
+            if t is not None:
+                p.time = t
         if U is None:
             U = initial_condition(self.params, self.geo.pos_spts)
         self.set_U(U)
@@ -278,6 +287,15 @@ class Solver:
         nat.hip_check(nat.hip_lib().fhip_check_state(self._ctx))
 
     # -- timing ------------------------------------------------------------------------
+    def writeData(self, directory: str = ".") -> Path:
+        """writeData (src/output.cpp:43-58): CSV (plotType 0) or ParaView (1)."""
+        from .output import vtu_name, write_csv, write_vtu
+        p = self.params
+        U = self.get_U()
+        if getattr(p, "plotType", 1) == 0:
+            return write_csv(Path(directory) / f"{p.dataFileName}.csv.{self.iter:09d}", self.host, U)
+        return write_vtu(Path(directory) / vtu_name(p.dataFileName, self.iter), self.host, U, p.time, self.iter)
+
     def autotune(self, path: int = nat.PATH_FUSED) -> list[int]:
         """Pick the block size of every fused kernel by timing them on the current state."""
         out = np.zeros(5, dtype=np.int32)
@@ -372,18 +390,23 @@ class MultiGrid:
             pass
 
 
-def run(params: InputParams, log=print) -> Solver:
-    """The main loop of src/flurry.cpp:120-155 (residual monitor included)."""
+def run(params: InputParams, log=print, output_dir: str | None = ".") -> Solver:
+    """The main loop of src/flurry.cpp:120-155 (residual monitor and plot
+    files included; output_dir=None writes no files)."""
     s = Solver(params)
     s.initializeSolution()
     p = params
     mg = MultiGrid(s) if getattr(p, "PMG", 0) else None
+    if output_dir is not None:
+        s.writeData(output_dir)
     while s.iter < p.iterMax and p.time < p.maxTime:
         s.update(1)
         if mg is not None:
             mg.cycle()
         p.time += s.dt
         it = s.iter
+        if output_dir is not None and (it % p.plotFreq == 0 or it == p.iterMax or p.time >= p.maxTime):
+            s.writeData(output_dir)
         if it % p.monitorResFreq == 0 or it == p.initIter + 1:
             dt = s.dt
             if dt < 1e-13:
