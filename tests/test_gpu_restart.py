@@ -9,7 +9,7 @@ import numpy as np
 import pytest
 
 from flurrypp_amd import cases
-from flurrypp_amd.solver import Solver
+from flurrypp_amd.solver import HostSetup, Solver
 
 pytestmark = pytest.mark.gpu
 
@@ -33,7 +33,7 @@ def test_restart_continues_the_run(tmp_path, make):
     q = copy.copy(p)
     q.restart, q.restartIter, q.initIter = 1, it, it
     q.time = 0.0
-    b = Solver(q, setup=setup)
+    b = Solver(q, setup=HostSetup(q, setup.mesh))
     b.initializeSolution(directory=tmp_path)
     assert q.time == pytest.approx(p.time, rel=1e-15)
     b.update(10)
