@@ -47,7 +47,6 @@ class FhipDesc(C.Structure):
         ("face_rel_rot", _ip), ("face_bc", _ip),
         ("n_meshes", C.c_int), ("mesh_ele_start", _ip),
         ("n_bnd_states", C.c_int), ("bnd_states", _dp), ("face_bnd_state", _ip),
-        ("ele_nodes", _dp), ("dshape_spts", _dp),
     ]
 
 
@@ -58,7 +57,6 @@ ARR = {
     "face_fpt_L": 12, "face_fpt_R": 13,
 }
 PATH_AUTO, PATH_STAGED, PATH_FUSED, PATH_MIXED = 0, 1, 2, 3
-OPT_GRAPHS, OPT_METRICS_FROM_NODES = 1, 2
 
 
 def dptr(a: np.ndarray | None):
@@ -107,7 +105,6 @@ def hip_lib() -> C.CDLL:
         lib.fhip_update.argtypes = [C.c_void_p, C.c_int, C.c_int]
         lib.fhip_path_supported.argtypes = [C.c_void_p, C.c_int]
         lib.fhip_set_option.argtypes = [C.c_void_p, C.c_int, C.c_int]
-        lib.fhip_get_option.argtypes = [C.c_void_p, C.c_int, _ip]
         lib.fhip_residual_norm.argtypes = [C.c_void_p, C.c_int, _dp]
         lib.fhip_mesh_residual_norms.argtypes = [C.c_void_p, C.c_int, _dp]
         lib.fhip_set_mesh_active.argtypes = [C.c_void_p, _ip]

@@ -77,8 +77,6 @@ class MergedSetup:
         self.bnd_states = np.ascontiguousarray(
             [[m.setup.params.rhoBound, m.setup.params.uBound, m.setup.params.vBound,
               getattr(m.setup.params, "wBound", 0.0), m.setup.params.pBound] for m in members], dtype=np.float64)
-        self.ele_nodes = np.ascontiguousarray(np.concatenate([m.setup.ele_nodes for m in members], axis=0))
-        self.dshape = s0.dshape
         fp = nat.FhipParams.from_buffer_copy(s0.fp)
         fp.n_eles = self.nEles
         self.fp = fp
@@ -94,7 +92,7 @@ class BatchGroup:
         self._keep = [o.spts_to_fpts, o.grad_spts, o.extrap_fn, o.correction, o.correct_u, o.w_spts, o.w_face,
                       g.jginv_spts, g.detjac_spts, g.norm_fpts, g.da_fpts,
                       f.eleL, f.locL, f.eleR, f.locR, f.relRot, f.bc, merged.ele_start, merged.bnd_states,
-                      merged.face_bnd_state, merged.ele_nodes, merged.dshape]
+                      merged.face_bnd_state]
         d = nat.FhipDesc()
         d.params = C.pointer(merged.fp)
         (d.opp_spts_to_fpts, d.opp_grad_spts, d.opp_extrapolate_fn, d.opp_correction, d.opp_correct_u,
@@ -108,7 +106,6 @@ class BatchGroup:
         d.n_bnd_states = len(merged.members)
         d.bnd_states = nat.dptr(merged.bnd_states)
         d.face_bnd_state = nat.iptr(merged.face_bnd_state)
-        d.ele_nodes, d.dshape_spts = nat.dptr(merged.ele_nodes), nat.dptr(merged.dshape)
         self._desc = d
         h = C.c_void_p()
         nat.hip_check(nat.hip_lib().fhip_create(C.byref(d), C.byref(h)))

@@ -758,23 +758,7 @@ int fhip_get_block_sizes(fhip_ctx* h, int path, int* block_sizes) {
 int fhip_set_option(fhip_ctx* h, int option, int value) {
   return guard([&] {
     require(h, "null context");
-    if (option == FHIP_OPT_GRAPHS) {
-      h->c.use_graphs = value != 0;
-    } else if (option == FHIP_OPT_METRICS_FROM_NODES) {
-      require(h->c.fused_ok, "metric terms from the vertices need the fused path");
-      fhip::fused_set_nodes(h->c, value != 0);
-    } else {
-      throw ApiError(FHIP_EINVAL, "unknown option");
-    }
-    return 0;
-  });
-}
-
-int fhip_get_option(fhip_ctx* h, int option, int* value) {
-  return guard([&] {
-    require(h && value, "null argument");
-    if (option == FHIP_OPT_GRAPHS) *value = h->c.use_graphs ? 1 : 0;
-    else if (option == FHIP_OPT_METRICS_FROM_NODES) *value = (h->c.fused_ok && fhip::fused_nodes(h->c)) ? 1 : 0;
+    if (option == FHIP_OPT_GRAPHS) h->c.use_graphs = value != 0;
     else throw ApiError(FHIP_EINVAL, "unknown option");
     return 0;
   });

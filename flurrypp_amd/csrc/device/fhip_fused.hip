@@ -123,10 +123,6 @@ struct FusedState {
   // P-multigrid source term (element-major copy of src_spts), used while use_src
   double* srce = nullptr;
   bool use_src = false;
-  // metric terms from the vertices: [E][NN][D] vertices, [D][S][NN] shape
-  // derivatives; used while use_nodes (verified at set-up)
-  double *xn = nullptr, *dsh = nullptr;
-  bool nodes_ok = false, use_nodes = false;
   // block size of each kernel (kExtrap..kRk) for FP64 [0] and mixed [1]
   int nt[2][5] = {{256, 256, 256, 256, 256}, {256, 256, 256, 256, 256}};
   Work<double> w64;
@@ -135,8 +131,7 @@ struct FusedState {
     w64.release();
     w32.release();
     void* ps[] = {tab_mem, Ue,  U0e,    Uacc,   divF0e,    JGe,    detJe, norme, dAe, waveSpe,
-                  pL,      pR,  bc,     bst,    meshDt,    meshRes, eleMesh, meshStart, active, srce,
-                  xn,      dsh};
+                  pL,      pR,  bc,     bst,    meshDt,    meshRes, eleMesh, meshStart, active, srce};
     for (void* p : ps)
       if (p) (void)hipFree(p);
   }
@@ -169,69 +164,6 @@ __global__ void k_from_em(long long total, int P, int E, int K, const double* __
   long long e = r % E;
   int p = (int)(r / E);
   out[i] = in[(e * K + k) * P + p];
-}
-
-// ---------------------------------------------------------------------------
-// JGinv (the adjugate of dx/dxi) and detJ at solution point s of one linear
-// element from its vertices xn [NN][D] and the shape derivatives dsh
-// [D][S][NN]: calcTransforms (src/solver.cpp:1587-1694) term by term, without
-// contraction, so the values equal the host-computed arrays bit for bit.
-template <int D>
-__device__ __forceinline__ void metric_at(const double* __restrict__ xn, const double* __restrict__ dsh, int S, int s,
-                                          double G[D][D], double& det) {
-#pragma clang fp contract(off)
-  constexpr int NN = D == 2 ? 4 : 8;
-  double J[D][D];
-#pragma unroll
-  for (int r = 0; r < D; r++)
-#pragma unroll
-    for (int x = 0; x < D; x++) {
-      double acc = 0;
-#pragma unroll
-      for (int n = 0; n < NN; n++) acc += dsh[(r * S + s) * NN + n] * xn[n * D + x];
-      J[r][x] = acc;
-    }
-  if constexpr (D == 2) {
-    det = J[0][0] * J[1][1] - J[0][1] * J[1][0];
-    G[0][0] = J[1][1];
-    G[0][1] = -J[1][0];
-    G[1][0] = -J[0][1];
-    G[1][1] = J[0][0];
-  } else {
-    const double xr = J[0][0], xs = J[1][0], xt = J[2][0];
-    const double yr = J[0][1], ys = J[1][1], yt = J[2][1];
-    const double zr = J[0][2], zs = J[1][2], zt = J[2][2];
-    det = xr * (ys * zt - yt * zs) - xs * (yr * zt - yt * zr) + xt * (yr * zs - ys * zr);
-    G[0][0] = ys * zt - yt * zs;
-    G[0][1] = xt * zs - xs * zt;
-    G[0][2] = xs * yt - xt * ys;
-    G[1][0] = yt * zr - yr * zt;
-    G[1][1] = xr * zt - xt * zr;
-    G[1][2] = xt * yr - xr * yt;
-    G[2][0] = yr * zs - ys * zr;
-    G[2][1] = xs * zr - xr * zs;
-    G[2][2] = xr * ys - xs * yr;
-  }
-}
-
-// Counts the solution points where the recomputed metric terms differ from
-// the stored ones (element-major JGe, detJe).
-template <int D>
-__global__ void k_check_metrics(int E, int S, const double* __restrict__ xn, const double* __restrict__ dsh,
-                                const double* __restrict__ JGe, const double* __restrict__ detJe, int* __restrict__ bad) {
-  const long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x;
-  if (i >= (long long)E * S) return;
-  const long long e = i / S;
-  const int s = (int)(i - e * S);
-  constexpr int NN = D == 2 ? 4 : 8;
-  double G[D][D], det;
-  metric_at<D>(xn + e * NN * D, dsh, S, s, G, det);
-  bool same = det == detJe[e * S + s];
-#pragma unroll
-  for (int r = 0; r < D; r++)
-#pragma unroll
-    for (int x = 0; x < D; x++) same = same && G[r][x] == JGe[((e * D + r) * D + x) * S + s];
-  if (!same) atomicAdd(bad, 1);
 }
 
 // ---------------------------------------------------------------------------
@@ -345,8 +277,7 @@ template <int D, int P, bool VISC, class R>
 __global__ __launch_bounds__(kMaxBlock) void k_volume(int E, const double* __restrict__ Ue, const R* __restrict__ dUce,
                                                  const double* __restrict__ JGe, const double* __restrict__ detJe,
                                                  R* __restrict__ divFe, R* __restrict__ disFne, R* __restrict__ dUfe,
-                                                 Tabs tb, Phys ph, const double* __restrict__ xn,
-                                                 const double* __restrict__ dsh) {
+                                                 Tabs tb, Phys ph) {
   using Sh = Shape<D, P>;
   constexpr int S = Sh::S, F = Sh::F, N = Sh::N, P1 = Sh::P1, NC = Sh::NC;
   const int EB = blockDim.x / S;
@@ -364,23 +295,11 @@ __global__ __launch_bounds__(kMaxBlock) void k_volume(int E, const double* __res
   // the point's metric terms: inviscid runs load them while the slab is
   // staged; viscous runs (short of registers) load them where they are used
   R G[D][D] = {};
-  double det = 1;
-  // from the element's vertices (xn given) or from the stored arrays
   auto load_G = [&] {
-    if (xn) {
-      double Gd[D][D];
-      metric_at<D>(xn + e * (D == 2 ? 4 : 8) * D, dsh, S, s, Gd, det);
 #pragma unroll
-      for (int r = 0; r < D; r++)
+    for (int r = 0; r < D; r++)
 #pragma unroll
-        for (int x = 0; x < D; x++) G[r][x] = R(Gd[r][x]);
-    } else {
-#pragma unroll
-      for (int r = 0; r < D; r++)
-#pragma unroll
-        for (int x = 0; x < D; x++) G[r][x] = R(JGe[((e * D + r) * D + x) * S + s]);
-      det = detJe[e * S + s];
-    }
+      for (int x = 0; x < D; x++) G[r][x] = R(JGe[((e * D + r) * D + x) * S + s]);
   };
   if constexpr (!VISC) {
     if (act) load_G();
@@ -427,7 +346,7 @@ __global__ __launch_bounds__(kMaxBlock) void k_volume(int E, const double* __res
     __syncthreads();  // sC is overwritten by the gradient below
     if (act) {
       load_G();
-      const R inv = R(1. / det);
+      const R inv = R(1. / detJe[e * S + s]);
 #pragma unroll
       for (int k = 0; k < N; k++)
 #pragma unroll
@@ -597,8 +516,7 @@ __global__ __launch_bounds__(kMaxBlock) void k_rk(int E, int mode, double a, dou
                                              double* __restrict__ U0e, double* __restrict__ Uacc,
                                              double* __restrict__ divF0e, R* __restrict__ Ufe, Tabs tb,
                                              const int* __restrict__ eleMesh, const double* __restrict__ meshDt,
-                                             const int* __restrict__ active, const double* __restrict__ srce,
-                                             const double* __restrict__ xn, const double* __restrict__ dsh) {
+                                             const int* __restrict__ active, const double* __restrict__ srce) {
   using Sh = Shape<D, P>;
   constexpr int S = Sh::S, F = Sh::F, N = Sh::N, P1 = Sh::P1, NC = Sh::NC;
   const int EB = blockDim.x / S;
@@ -630,13 +548,7 @@ __global__ __launch_bounds__(kMaxBlock) void k_rk(int E, int mode, double a, dou
     // dt: 0 global, 1 per element (local CFL), 2 per mesh (batched global CFL)
     const double dt = (local == 1) ? eleDt[e] : (local == 2 ? meshDt[eleMesh[e]] : *dtp);
     const bool frozen = active && !active[eleMesh[e]];
-    double dj;
-    if (xn) {
-      double Gd[D][D];
-      metric_at<D>(xn + e * (D == 2 ? 4 : 8) * D, dsh, S, s, Gd, dj);
-    } else {
-      dj = detJe[e * S + s];
-    }
+    const double dj = detJe[e * S + s];
 #pragma unroll
     for (int k = 0; k < N; k++) {
       const long long iu = (e * N + k) * S + s;
@@ -953,36 +865,6 @@ bool fused_setup(Ctx& c, const fhip_desc* d) {
     FCHK(hipMemcpy(st->norme, nr.data(), nr.size() * 8, hipMemcpyHostToDevice));
     FCHK(hipMemcpy(st->dAe, da.data(), da.size() * 8, hipMemcpyHostToDevice));
   }
-  // metric terms from the vertices: kept only if they reproduce the arrays
-  if (d->ele_nodes && d->dshape_spts) {
-    const long long NN = D == 2 ? 4 : 8;
-    st->xn = dal<double>(E * NN * D);
-    st->dsh = dal<double>(D * S * NN);
-    int* bad = dal<int>(1);
-    FCHK(hipMemcpy(st->xn, d->ele_nodes, (size_t)E * NN * D * 8, hipMemcpyHostToDevice));
-    FCHK(hipMemcpy(st->dsh, d->dshape_spts, (size_t)D * S * NN * 8, hipMemcpyHostToDevice));
-    FCHK(hipMemsetAsync(bad, 0, 4, c.stream));
-    if (D == 2)
-      hipLaunchKernelGGL(k_check_metrics<2>, dim3(nblk(E * S, 256)), dim3(256), 0, c.stream, z.E, z.S, st->xn, st->dsh,
-                         st->JGe, st->detJe, bad);
-    else
-      hipLaunchKernelGGL(k_check_metrics<3>, dim3(nblk(E * S, 256)), dim3(256), 0, c.stream, z.E, z.S, st->xn, st->dsh,
-                         st->JGe, st->detJe, bad);
-    FCHK(hipGetLastError());
-    int h = -1;
-    FCHK(hipMemcpyAsync(&h, bad, 4, hipMemcpyDeviceToHost, c.stream));
-    FCHK(hipStreamSynchronize(c.stream));
-    (void)hipFree(bad);
-    // verified, but off by default: on MI355X the recomputation costs the
-    // latency-bound volume and RK kernels more than the bytes it saves
-    st->nodes_ok = (h == 0);
-    st->use_nodes = false;
-    if (!st->nodes_ok) {
-      (void)hipFree(st->xn);
-      (void)hipFree(st->dsh);
-      st->xn = st->dsh = nullptr;
-    }
-  }
   // batching: meshes, boundary-state table
   if (d->n_meshes > 1 || d->mesh_ele_start) {
     int nm = std::max(1, d->n_meshes);
@@ -1071,16 +953,6 @@ void fused_set_active(Ctx& c, const int* active) {
   FCHK(hipStreamSynchronize(c.stream));
 }
 
-bool fused_set_nodes(Ctx& c, bool on) {
-  FusedState& st = fs(c);
-  const bool v = on && st.nodes_ok;
-  if (v != st.use_nodes) c.drop_graphs();
-  st.use_nodes = v;
-  return v;
-}
-
-bool fused_nodes(Ctx& c) { return fs(c).use_nodes; }
-
 void fused_release(Ctx& c) {
   if (c.fused) delete reinterpret_cast<FusedState*>(c.fused);
   c.fused = nullptr;
@@ -1130,12 +1002,10 @@ struct Runner {
     const bool visc = c.prm.viscous != 0;
     if (visc)
       hipLaunchKernelGGL((k_volume<D, P, true, R>), dim3(nblk(c.z.E, nt / S)), dim3(nt), lds(kVolume, nt, true),
-                         c.stream, c.z.E, st.Ue, w.dUce, st.JGe, st.detJe, w.divFe, w.disFne, w.dUfe, st.tb, c.ph,
-                         st.use_nodes ? st.xn : nullptr, st.dsh);
+                         c.stream, c.z.E, st.Ue, w.dUce, st.JGe, st.detJe, w.divFe, w.disFne, w.dUfe, st.tb, c.ph);
     else
       hipLaunchKernelGGL((k_volume<D, P, false, R>), dim3(nblk(c.z.E, nt / S)), dim3(nt), lds(kVolume, nt, false),
-                         c.stream, c.z.E, st.Ue, w.dUce, st.JGe, st.detJe, w.divFe, w.disFne, w.dUfe, st.tb, c.ph,
-                         st.use_nodes ? st.xn : nullptr, st.dsh);
+                         c.stream, c.z.E, st.Ue, w.dUce, st.JGe, st.detJe, w.divFe, w.disFne, w.dUfe, st.tb, c.ph);
     FCHK(hipGetLastError());
   }
   static void launch_face_visc(Ctx& c, int nt) {
@@ -1154,7 +1024,7 @@ struct Runner {
                        a, b, w.divFe, w.Fne, w.disFne, st.detJe, c.d_dt, c.eleDt,
                        c.prm.dt_type == 2 ? 1 : (c.prm.dt_type == 1 && st.nMesh > 1 ? 2 : 0), st.Ue, st.U0e, st.Uacc,
                        write0 ? st.divF0e : nullptr, w.Ufe, st.tb, st.eleMesh, st.meshDt, st.active,
-                       st.use_src ? st.srce : nullptr, st.use_nodes ? st.xn : nullptr, st.dsh);
+                       st.use_src ? st.srce : nullptr);
     FCHK(hipGetLastError());
   }
   static void extrap(Ctx& c) {

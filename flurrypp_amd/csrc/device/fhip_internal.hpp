@@ -162,9 +162,6 @@ void fused_residual(Ctx& c, bool mixed);  // divF_spts[0] and wave speeds of the
 void fused_autotune(Ctx& c, bool mixed, int* out);
 void fused_set_blocks(Ctx& c, bool mixed, const int* nt);
 void fused_get_blocks(Ctx& c, bool mixed, int* out);
-// metric terms recomputed from the vertices (when verified at set-up)
-bool fused_set_nodes(Ctx& c, bool on);
-bool fused_nodes(Ctx& c);
 void fused_mesh_residuals(Ctx& c, int type, double* out);  // out [nMesh][N]
 int fused_n_meshes(Ctx& c);
 void fused_set_active(Ctx& c, const int* active);

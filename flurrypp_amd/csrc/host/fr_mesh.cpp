@@ -892,23 +892,6 @@ int fhost_mesh_get_faces(const fhost_mesh* m, int* eleL, int* locL, int* eleR, i
   return 0;
 }
 
-int fhost_dshape_spts(int nDims, int order, int ptsType, double* out) {
-  return guard([&] {
-    if (nDims != 2 && nDims != 3) throw Error("nDims must be 2 or 3");
-    int S, F, Ff;
-    fr::sizes(nDims, order, S, F, Ff);
-    std::vector<fr::Pt> ls, lf;
-    fr::ref_points(nDims, order, ptsType, ls, lf);
-    const int NN = nDims == 2 ? 4 : 8;
-    std::vector<double> tmp((size_t)nDims * NN);
-    for (int s = 0; s < S; s++) {
-      shape_lin(nDims, ls[s], nullptr, tmp.data());
-      for (int d = 0; d < nDims; d++)
-        for (int n = 0; n < NN; n++) out[((size_t)d * S + s) * NN + n] = tmp[d * NN + n];
-    }
-  });
-}
-
 int fhost_mesh_geometry(const fhost_mesh* m, int order, int ptsType, double* pos_spts, double* pos_fpts,
                         double* jginv_spts, double* detjac_spts, double* norm_fpts, double* da_fpts) {
   return guard([&] {

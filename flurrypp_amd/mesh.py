@@ -64,14 +64,6 @@ def pmg_operators(nDims: int, order: int, spts_type: str = "Legendre"):
     return pro, res
 
 
-def dshape_spts(nDims: int, order: int, spts_type: str = "Legendre") -> np.ndarray:
-    """Linear shape-function derivatives at the solution points, [D][S][2^D]."""
-    S = (order + 1) ** nDims
-    out = np.zeros((nDims, S, 2 ** nDims))
-    host_check(host_lib().fhost_dshape_spts(nDims, order, PTS[spts_type.lower()], dptr(out)))
-    return out
-
-
 def pts1d(order: int, spts_type: str = "Legendre") -> np.ndarray:
     out = np.zeros(order + 1)
     host_check(host_lib().fhost_pts1d(PTS[spts_type.lower()], order, dptr(out)))

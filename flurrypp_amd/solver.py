@@ -17,7 +17,7 @@ import numpy as np
 
 from . import _native as nat
 from .input import InputParams
-from .mesh import Mesh, bc_number, dshape_spts, operators, pmg_operators, sizes
+from .mesh import Mesh, bc_number, operators, pmg_operators, sizes
 
 PI = 4.0 * math.atan(1.0)  # include/global.hpp:72
 
@@ -124,11 +124,6 @@ class HostSetup:
         self.geo = self.mesh.geometry(self.order, p.sptsTypeQuad)
         self.faces = self.mesh.faces()
         self.fp = fhip_params(p, self.nEles)
-        # vertices per element and shape derivatives: the fused kernels rebuild
-        # the metric terms from them
-        xv, c2v = self.mesh.nodes()
-        self.ele_nodes = np.ascontiguousarray(xv[c2v[:, : 2 ** self.nDims]])
-        self.dshape = dshape_spts(self.nDims, self.order, p.sptsTypeQuad)
 
     def initial_condition(self) -> np.ndarray:
         return initial_condition(self.params, self.geo.pos_spts)
@@ -153,7 +148,7 @@ class Solver:
         o, g = self.opers, self.geo
         self._keep = [o.spts_to_fpts, o.grad_spts, o.extrap_fn, o.correction, o.correct_u, o.w_spts, o.w_face,
                       g.jginv_spts, g.detjac_spts, g.norm_fpts, g.da_fpts,
-                      f.eleL, f.locL, f.eleR, f.locR, f.relRot, f.bc, h.ele_nodes, h.dshape]
+                      f.eleL, f.locL, f.eleR, f.locR, f.relRot, f.bc]
         d = nat.FhipDesc()
         d.params = C.pointer(self._fp)
         (d.opp_spts_to_fpts, d.opp_grad_spts, d.opp_extrapolate_fn, d.opp_correction, d.opp_correct_u,
@@ -161,8 +156,7 @@ class Solver:
         d.jginv_spts, d.detjac_spts, d.norm_fpts, d.da_fpts = (nat.dptr(a) for a in self._keep[7:11])
         d.n_faces = len(f.eleL)
         (d.face_ele_l, d.face_loc_l, d.face_ele_r, d.face_loc_r, d.face_rel_rot,
-         d.face_bc) = (nat.iptr(a) for a in self._keep[11:17])
-        d.ele_nodes, d.dshape_spts = nat.dptr(h.ele_nodes), nat.dptr(h.dshape)
+         d.face_bc) = (nat.iptr(a) for a in self._keep[11:])
         self._desc = d
         lib = nat.hip_lib()
         hh = C.c_void_p()

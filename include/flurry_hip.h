@@ -124,17 +124,6 @@ typedef struct fhip_desc {
   int n_bnd_states;
   const double* bnd_states;         /* [n_bnd_states][5] */
   const int* face_bnd_state;        /* [n_faces] */
-  /* Optional, linear quads/hexes only (both or neither): the element vertices
-   * and the shape-function derivatives at the solution points (calcTransforms,
-   * src/solver.cpp:1587-1694).  With them, and FHIP_OPT_METRICS_FROM_NODES set,
-   * the fused paths recompute JGinv_spts and detJac_spts from the vertices
-   * inside their kernels instead of reading them (80 bytes per solution point
-   * and stage in 3-D).  fhip_create checks on the device that the
-   * recomputation reproduces the arrays above bit for bit; the option is
-   * refused otherwise.  Off by default: on MI355X the extra arithmetic costs
-   * the (latency-bound) volume and RK kernels more than the bytes save. */
-  const double* ele_nodes;          /* [n_eles][2^nDims][nDims], FlurryPP node order */
-  const double* dshape_spts;        /* [nDims][nSpts][2^nDims]: d shape_n / d xi_r */
 } fhip_desc;
 
 /* Array ids for fhip_get_array / fhip_set_array / fhip_device_ptr. */
@@ -195,13 +184,9 @@ int fhip_sync(fhip_ctx* ctx);
 int fhip_path_supported(fhip_ctx* ctx, int path);
 
 /* Options: FHIP_OPT_GRAPHS (1, default on) replays the fused path's RK step
- * as one captured HIP graph.  FHIP_OPT_METRICS_FROM_NODES (2, default off;
- * takes effect only when ele_nodes/dshape_spts were given and verified)
- * recomputes the metric terms from the vertices.  fhip_get_option reads
- * either (the effective value). */
-enum { FHIP_OPT_GRAPHS = 1, FHIP_OPT_METRICS_FROM_NODES = 2 };
+ * as one captured HIP graph. */
+enum { FHIP_OPT_GRAPHS = 1 };
 int fhip_set_option(fhip_ctx* ctx, int option, int value);
-int fhip_get_option(fhip_ctx* ctx, int option, int* value);
 
 /* Block sizes (threads per block, 64..512) of the fused kernels, in the order
  * extrapolate, face_inviscid, volume, face_viscous, rk_update, for path

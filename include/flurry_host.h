@@ -81,11 +81,6 @@ int fhost_operators(int nDims, int order, int ptsType, int vcjh,
  * Either pointer may be NULL. */
 int fhost_pmg_operators(int nDims, int order, int ptsType, double* prolong, double* restrict_op);
 
-/* Derivatives of the linear shape functions at the solution points,
- * out[nDims][nSpts][2^nDims] (d shape_n / d xi_r, FlurryPP node order), as
- * calcTransforms uses them (src/solver.cpp:1587-1694). */
-int fhost_dshape_spts(int nDims, int order, int ptsType, double* out);
-
 /* ---- meshes ------------------------------------------------------------- */
 
 /* Cartesian mesh with createMesh's vertex, element and boundary numbering
