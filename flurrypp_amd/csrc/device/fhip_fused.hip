@@ -78,7 +78,17 @@ struct Tabs {
   const double* cor_w;  // [S][NC]    opp_correction entries
   const int* cu_idx;    // [D][S][2]  flux points with opp_correct_u[d][s][f] != 0
   const double* cu_w;   // [D][S][2]  their opp_correct_u entries
+  // the same weights rounded to FP32 once, for the mixed path
+  const float *ext_wf, *efn_wf, *grd_wf, *cor_wf, *cu_wf;
 };
+
+// Operator weights in the kernels' arithmetic type.
+template <class R>
+__device__ __forceinline__ const R* tw(const double* d, const float* f);
+template <>
+__device__ __forceinline__ const double* tw<double>(const double* d, const float*) { return d; }
+template <>
+__device__ __forceinline__ const float* tw<float>(const double*, const float* f) { return f; }
 
 // Position of solution point s = i + P1*(j + P1*k) in an element's LDS slab.
 // Each (j,k) row is rotated by j+k and each k plane by k, so that the P1^2
@@ -188,7 +198,7 @@ __global__ __launch_bounds__(kMaxBlock) void k_extrap(int E, const double* __res
     for (int k = 0; k < N; k++) {
       R acc = 0;
 #pragma unroll
-      for (int m = 0; m < P1; m++) acc += R(tb.ext_w[f * P1 + m]) * sU[(b * N + k) * S + tb.ext_idx[f * P1 + m]];
+      for (int m = 0; m < P1; m++) acc += tw<R>(tb.ext_w, tb.ext_wf)[f * P1 + m] * sU[(b * N + k) * S + tb.ext_idx[f * P1 + m]];
       Ufe[((long long)(e0 + b) * N + k) * F + f] = acc;
     }
   }
@@ -327,18 +337,18 @@ __global__ __launch_bounds__(kMaxBlock) void k_volume(int E, const double* __res
 #pragma unroll
       for (int d = 0; d < D; d++) {
         const int* gi = tb.grd_idx + (d * S + s) * P1;
-        const double* gw = tb.grd_w + (d * S + s) * P1;
+        const R* gw = tw<R>(tb.grd_w, tb.grd_wf) + (d * S + s) * P1;
         const int* ci = tb.cu_idx + (d * S + s) * 2;
-        const double* cw = tb.cu_w + (d * S + s) * 2;
+        const R* cw = tw<R>(tb.cu_w, tb.cu_wf) + (d * S + s) * 2;
 #pragma unroll 1
         for (int k = 0; k < N; k++) {
           R a = 0;
 #pragma unroll
-          for (int m = 0; m < P1; m++) a += R(gw[m]) * sU[(b * N + k) * S + gi[m]];
+          for (int m = 0; m < P1; m++) a += gw[m] * sU[(b * N + k) * S + gi[m]];
           // the other correction entries of the row are exactly zero
           R c = 0;
 #pragma unroll
-          for (int j = 0; j < 2; j++) c += R(cw[j]) * sC[(b * N + k) * F + ci[j]];
+          for (int j = 0; j < 2; j++) c += cw[j] * sC[(b * N + k) * F + ci[j]];
           gr[d][k] = c + a;
         }
       }
@@ -362,14 +372,14 @@ __global__ __launch_bounds__(kMaxBlock) void k_volume(int E, const double* __res
       const int bb = i / F, f = i - bb * F;
       const long long ee = e0 + bb;
       const int* xi = tb.ext_idx + f * P1;
-      const double* xw = tb.ext_w + f * P1;
+      const R* xw = tw<R>(tb.ext_w, tb.ext_wf) + f * P1;
 #pragma unroll 1
       for (int x = 0; x < D; x++)
 #pragma unroll
         for (int k = 0; k < N; k++) {
           R a = 0;
 #pragma unroll
-          for (int m = 0; m < P1; m++) a += R(xw[m]) * sG[((bb * D + x) * N + k) * S + xi[m]];
+          for (int m = 0; m < P1; m++) a += xw[m] * sG[((bb * D + x) * N + k) * S + xi[m]];
           dUfe[((ee * D + x) * N + k) * F + f] = a;
         }
     }
@@ -417,10 +427,10 @@ __global__ __launch_bounds__(kMaxBlock) void k_volume(int E, const double* __res
 #pragma unroll
       for (int d = 0; d < D; d++) {
         const int* gi = tb.grd_idx + (d * S + s) * P1;
-        const double* gw = tb.grd_w + (d * S + s) * P1;
+        const R* gw = tw<R>(tb.grd_w, tb.grd_wf) + (d * S + s) * P1;
         R a = 0;
 #pragma unroll
-        for (int m = 0; m < P1; m++) a += R(gw[m]) * sF[((b * D + d) * N + k) * S + gi[m]];
+        for (int m = 0; m < P1; m++) a += gw[m] * sF[((b * D + d) * N + k) * S + gi[m]];
         tot = (d == 0) ? a : a + tot;
       }
       divFe[(e * N + k) * S + s] = tot;
@@ -434,12 +444,12 @@ __global__ __launch_bounds__(kMaxBlock) void k_volume(int E, const double* __res
     // extrapolateNormalFlux: of the sum over directions only the face's own
     // normal direction is non-zero (the other terms are exact zeros)
     const int dn = tb.efn_dir[f];
-    const double* ew = tb.efn_w + f * P1;
+    const R* ew = tw<R>(tb.efn_w, tb.efn_wf) + f * P1;
 #pragma unroll
     for (int k = 0; k < N; k++) {
       R a = 0;
 #pragma unroll
-      for (int m = 0; m < P1; m++) a += R(ew[m]) * sF[((bb * D + dn) * N + k) * S + xi[m]];
+      for (int m = 0; m < P1; m++) a += ew[m] * sF[((bb * D + dn) * N + k) * S + xi[m]];
       disFne[(ee * N + k) * F + f] = a;
     }
   }
@@ -537,7 +547,7 @@ __global__ __launch_bounds__(kMaxBlock) void k_rk(int E, int mode, double a, dou
       for (int k = 0; k < N; k++) {
         R c = 0;
 #pragma unroll
-        for (int j = 0; j < NC; j++) c += R(tb.cor_w[s * NC + j]) * sD[(b * N + k) * F + tb.cor_idx[s * NC + j]];
+        for (int j = 0; j < NC; j++) c += tw<R>(tb.cor_w, tb.cor_wf)[s * NC + j] * sD[(b * N + k) * F + tb.cor_idx[s * NC + j]];
         divF0e[(e * N + k) * S + s] = double(c + divFe[(e * N + k) * S + s]);
       }
     }
@@ -554,7 +564,7 @@ __global__ __launch_bounds__(kMaxBlock) void k_rk(int E, int mode, double a, dou
       const long long iu = (e * N + k) * S + s;
       R c = 0;
 #pragma unroll
-      for (int j = 0; j < NC; j++) c += R(tb.cor_w[s * NC + j]) * sD[(b * N + k) * F + tb.cor_idx[s * NC + j]];
+      for (int j = 0; j < NC; j++) c += tw<R>(tb.cor_w, tb.cor_wf)[s * NC + j] * sD[(b * N + k) * F + tb.cor_idx[s * NC + j]];
       const double r0 = double(c + divFe[iu]);
       if (divF0e) divF0e[iu] = r0;
       // P-multigrid source: divF_spts[step] + src_spts (src/solver.cpp:480,514)
@@ -587,7 +597,7 @@ __global__ __launch_bounds__(kMaxBlock) void k_rk(int E, int mode, double a, dou
     for (int k = 0; k < N; k++) {
       R acc = 0;
 #pragma unroll
-      for (int m = 0; m < P1; m++) acc += R(tb.ext_w[f * P1 + m]) * sU[(bb * N + k) * S + tb.ext_idx[f * P1 + m]];
+      for (int m = 0; m < P1; m++) acc += tw<R>(tb.ext_w, tb.ext_wf)[f * P1 + m] * sU[(bb * N + k) * S + tb.ext_idx[f * P1 + m]];
       Ufe[((long long)(e0 + bb) * N + k) * F + f] = acc;
     }
   }
@@ -720,7 +730,7 @@ static T* dal(size_t n) {
 
 // Extract the non-zero structure of the dense operators.  Returns false if an
 // operator row has more non-zeros than the tensor-product structure allows.
-static bool build_tables(const Ctx& c, const fhip_desc* d, std::vector<char>& blob, size_t off[10]) {
+static bool build_tables(const Ctx& c, const fhip_desc* d, std::vector<char>& blob, size_t off[15]) {
   const int D = c.z.D, S = c.z.S, F = c.z.F, P1 = c.z.P + 1, NC = 2 * D;
   std::vector<int> ext_idx((size_t)F * P1, 0), efn_dir(F, 0), grd_idx((size_t)D * S * P1, 0),
       cor_idx((size_t)S * NC, 0), cu_idx((size_t)D * S * 2, 0);
@@ -801,6 +811,15 @@ static bool build_tables(const Ctx& c, const fhip_desc* d, std::vector<char>& bl
   put(cor_w.data(), cor_w.size() * 8, off[7]);
   put(cu_idx.data(), cu_idx.size() * 4, off[8]);
   put(cu_w.data(), cu_w.size() * 8, off[9]);
+  auto putf = [&](const std::vector<double>& v, size_t& o) {
+    std::vector<float> f(v.begin(), v.end());
+    put(f.data(), f.size() * 4, o);
+  };
+  putf(ext_w, off[10]);
+  putf(efn_w, off[11]);
+  putf(grd_w, off[12]);
+  putf(cor_w, off[13]);
+  putf(cu_w, off[14]);
   return true;
 }
 
@@ -825,7 +844,7 @@ bool fused_setup(Ctx& c, const fhip_desc* d) {
   const Sizes& z = c.z;
   if (z.P < 0 || z.P > 5) return false;
   std::vector<char> blob;
-  size_t off[10];
+  size_t off[15];
   if (!build_tables(c, d, blob, off)) return false;
   auto* st = new FusedState();
   c.fused = st;
@@ -835,7 +854,8 @@ bool fused_setup(Ctx& c, const fhip_desc* d) {
   st->tb = Tabs{(const int*)(base + off[0]),    (const double*)(base + off[1]), (const int*)(base + off[2]),
                 (const double*)(base + off[3]), (const int*)(base + off[4]),    (const double*)(base + off[5]),
                 (const int*)(base + off[6]),    (const double*)(base + off[7]), (const int*)(base + off[8]),
-                (const double*)(base + off[9])};
+                (const double*)(base + off[9]), (const float*)(base + off[10]), (const float*)(base + off[11]),
+                (const float*)(base + off[12]), (const float*)(base + off[13]), (const float*)(base + off[14])};
   const long long E = z.E, S = z.S, F = z.F, N = z.N, D = z.D;
   st->Ue = dal<double>(E * N * S);
   st->U0e = dal<double>(E * N * S);
