@@ -109,9 +109,10 @@ template <class R>
 struct Work {
   R *divFe = nullptr, *Ufe = nullptr, *Fne = nullptr, *disFne = nullptr, *dUce = nullptr, *dUfe = nullptr,
     *FnInv = nullptr;
+  R* Rst = nullptr;  // mixed path: residual of each classical RK stage but the last
   bool ready = false;
   void release() {
-    void* ps[] = {divFe, Ufe, Fne, disFne, dUce, dUfe, FnInv};
+    void* ps[] = {divFe, Ufe, Fne, disFne, dUce, dUfe, FnInv, Rst};
     for (void* p : ps)
       if (p) (void)hipFree(p);
     *this = Work<R>();
@@ -518,6 +519,13 @@ __global__ __launch_bounds__(kMaxBlock) void k_face_visc(int nFF, const int* __r
 // folded into one pass per stage, FP64) and U at the flux points for the
 // next stage.  mode: 0 first classical stage, 1 middle classical stage,
 // 2 last classical stage, 3 single-stage forward Euler, 4 Jameson-form stage.
+// Classical stages combine into U0 - sum_j b_j r_j / detJ dt in timeStepB's
+// order (src/solver.cpp:504-536): either through the FP64 accumulator Uacc, or
+// (mixed path, Rst given) from the stored stage residuals, which FP32 holds
+// exactly there since they are sums of FP32 values; both give the same bits.
+struct RkB {
+  double b[8];
+};
 template <int D, int P, class R>
 __global__ __launch_bounds__(kMaxBlock) void k_rk(int E, int mode, double a, double bcoef, const R* __restrict__ divFe,
                                              const R* __restrict__ Fne, const R* __restrict__ disFne,
@@ -526,10 +534,12 @@ __global__ __launch_bounds__(kMaxBlock) void k_rk(int E, int mode, double a, dou
                                              double* __restrict__ U0e, double* __restrict__ Uacc,
                                              double* __restrict__ divF0e, R* __restrict__ Ufe, Tabs tb,
                                              const int* __restrict__ eleMesh, const double* __restrict__ meshDt,
-                                             const int* __restrict__ active, const double* __restrict__ srce) {
+                                             const int* __restrict__ active, const double* __restrict__ srce,
+                                             R* __restrict__ Rst, int stage, RkB bs) {
   using Sh = Shape<D, P>;
   constexpr int S = Sh::S, F = Sh::F, N = Sh::N, P1 = Sh::P1, NC = Sh::NC;
   const int EB = blockDim.x / S;
+  const long long nS = (long long)E * N * S;
   R* const sU = reinterpret_cast<R*>(fhip_lds);  // [EB*N*S]
   R* const sD = sU + EB * N * S;                  // [EB*N*F] Fn - disFn at the block's flux points
   const int e0 = blockIdx.x * EB, nb = min(EB, E - e0);
@@ -573,13 +583,22 @@ __global__ __launch_bounds__(kMaxBlock) void k_rk(int E, int mode, double a, dou
       double un;
       if (mode == 0) {
         U0e[iu] = u;
-        Uacc[iu] = u - bcoef * r / dj * dt;
+        if (Rst) Rst[iu] = R(r);
+        else Uacc[iu] = u - bcoef * r / dj * dt;
         un = u - a * r / dj * dt;
       } else if (mode == 1) {
-        Uacc[iu] = Uacc[iu] - bcoef * r / dj * dt;
+        if (Rst) Rst[stage * nS + iu] = R(r);
+        else Uacc[iu] = Uacc[iu] - bcoef * r / dj * dt;
         un = U0e[iu] - a * r / dj * dt;
       } else if (mode == 2) {
-        un = Uacc[iu] - bcoef * r / dj * dt;
+        double acc;
+        if (Rst) {
+          acc = U0e[iu];
+          for (int j = 0; j < stage; j++) acc = acc - bs.b[j] * double(Rst[j * nS + iu]) / dj * dt;
+        } else {
+          acc = Uacc[iu];
+        }
+        un = acc - bcoef * r / dj * dt;
       } else if (mode == 3) {
         un = u - bcoef * r / dj * dt;
       } else {
@@ -837,6 +856,7 @@ static void ensure_work(Ctx& c) {
     w.dUfe = dal<R>(E * D * N * F);
     w.FnInv = dal<R>((long long)c.z.nFaceFpts * N);
   }
+  if (sizeof(R) == 4 && c.z.nRK > 1 && c.prm.time_type < 5) w.Rst = dal<R>((long long)(c.z.nRK - 1) * E * N * S);
   w.ready = true;
 }
 
@@ -1037,14 +1057,17 @@ struct Runner {
                        w.Ufe, w.dUfe, st.norme, st.dAe, w.Fne, w.FnInv, c.ph, st.bst);
     FCHK(hipGetLastError());
   }
-  static void launch_rk(Ctx& c, int nt, int mode, double a, double b, bool write0) {
+  static void launch_rk(Ctx& c, int nt, int mode, double a, double b, bool write0, int stage = 0) {
     FusedState& st = fs(c);
     Work<R>& w = st.work<R>();
+    RkB bs{};
+    for (int j = 0; j < c.z.nRK && j < 8; j++) bs.b[j] = c.prm.rk_b[j];
+    R* rst = (w.Rst && !st.use_src) ? w.Rst : nullptr;
     hipLaunchKernelGGL((k_rk<D, P, R>), dim3(nblk(c.z.E, nt / S)), dim3(nt), lds(kRk, nt, false), c.stream, c.z.E, mode,
                        a, b, w.divFe, w.Fne, w.disFne, st.detJe, c.d_dt, c.eleDt,
                        c.prm.dt_type == 2 ? 1 : (c.prm.dt_type == 1 && st.nMesh > 1 ? 2 : 0), st.Ue, st.U0e, st.Uacc,
                        write0 ? st.divF0e : nullptr, w.Ufe, st.tb, st.eleMesh, st.meshDt, st.active,
-                       st.use_src ? st.srce : nullptr);
+                       st.use_src ? st.srce : nullptr, rst, stage, bs);
     FCHK(hipGetLastError());
   }
   static void extrap(Ctx& c) {
@@ -1091,9 +1114,9 @@ struct Runner {
       launch_face_visc(c, blk(c, kFaceVisc));
     }
   }
-  static void rk(Ctx& c, int mode, double a, double b, bool write0) {
+  static void rk(Ctx& c, int mode, double a, double b, bool write0, int stage) {
     Launch L(c, nm("fused_rk_update", "mixed_rk_update"));
-    launch_rk(c, blk(c, kRk), mode, a, b, write0);
+    launch_rk(c, blk(c, kRk), mode, a, b, write0, stage);
   }
   // Time every valid block size of every kernel on this context's data and
   // keep the fastest.  The RK kernel runs in its middle-stage mode, which
@@ -1165,7 +1188,7 @@ struct Ops {
   void (*wave)(Ctx&);
   void (*dt)(Ctx&, double);
   void (*residual)(Ctx&);
-  void (*rk)(Ctx&, int, double, double, bool);
+  void (*rk)(Ctx&, int, double, double, bool, int);
   void (*ensure)(Ctx&);
   void (*tune)(Ctx&);
   bool (*check)(Ctx&, const int*);
@@ -1297,7 +1320,7 @@ void fused_residual(Ctx& c, bool mixed) {
   Ops o = get_ops(c, mixed);
   fused_begin(c, mixed, false);
   o.residual(c);
-  o.rk(c, 5, 0., 0., true);
+  o.rk(c, 5, 0., 0., true, 0);
   long long nS = (long long)z.E * z.N * z.S;
   hipLaunchKernelGGL(k_from_em, dim3(nblk(nS, 256)), dim3(256), 0, c.stream, nS, z.S, z.E, z.N, st.divF0e, c.divF);
   long long nW = (long long)z.E * z.F;
@@ -1321,7 +1344,7 @@ void fused_rk_step(Ctx& c, bool refresh, bool mixed) {
       if (refresh) o.wave(c);
       o.dt(c, cflfac);
     }
-    o.rk(c, jameson ? 4 : 3, A[1], B[0], true);
+    o.rk(c, jameson ? 4 : 3, A[1], B[0], true, 0);
     return;
   }
   if (cfl) {
@@ -1339,7 +1362,7 @@ void fused_rk_step(Ctx& c, bool refresh, bool mixed) {
     int mode;
     if (jameson) mode = 4;
     else mode = (step == 0) ? 0 : (step == nRK - 1 ? 2 : 1);
-    o.rk(c, mode, A[step + 1], B[step], step == 0);
+    o.rk(c, mode, A[step + 1], B[step], step == 0, step);
   }
 }
 
