@@ -285,10 +285,10 @@ __global__ __launch_bounds__(kMaxBlock) void k_face_inv(int nFF, const int* __re
 // [and gradient at faces].  One thread per solution point, EB elements per
 // block, element data staged in LDS.
 template <int D, int P, bool VISC, class R>
-__global__ __launch_bounds__(kMaxBlock) void k_volume(int E, const double* __restrict__ Ue, const R* __restrict__ dUce,
-                                                 const double* __restrict__ JGe, const double* __restrict__ detJe,
-                                                 R* __restrict__ divFe, R* __restrict__ disFne, R* __restrict__ dUfe,
-                                                 Tabs tb, Phys ph) {
+__device__ __forceinline__ void volume_body(int E, const double* __restrict__ Ue, const R* __restrict__ dUce,
+                                            const double* __restrict__ JGe, const double* __restrict__ detJe,
+                                            R* __restrict__ divFe, R* __restrict__ disFne, R* __restrict__ dUfe,
+                                            Tabs tb, Phys ph) {
   using Sh = Shape<D, P>;
   constexpr int S = Sh::S, F = Sh::F, N = Sh::N, P1 = Sh::P1, NC = Sh::NC;
   const int EB = blockDim.x / S;
@@ -334,7 +334,11 @@ __global__ __launch_bounds__(kMaxBlock) void k_volume(int E, const double* __res
     // src/solver.cpp:1105-1136), kept in LDS for phase 2 and the faces
     R* const sC = sF;
     R gr[D][N];
+    R inv = 0;
     if (act) {
+      // metric loads in flight while the reference gradient is formed
+      load_G();
+      inv = R(1. / detJe[e * S + s]);
 #pragma unroll
       for (int d = 0; d < D; d++) {
         const int* gi = tb.grd_idx + (d * S + s) * P1;
@@ -356,8 +360,6 @@ __global__ __launch_bounds__(kMaxBlock) void k_volume(int E, const double* __res
     }
     __syncthreads();  // sC is overwritten by the gradient below
     if (act) {
-      load_G();
-      const R inv = R(1. / detJe[e * S + s]);
 #pragma unroll
       for (int k = 0; k < N; k++)
 #pragma unroll
@@ -408,7 +410,6 @@ __global__ __launch_bounds__(kMaxBlock) void k_volume(int E, const double* __res
     } else {
       inviscid_flux<D, R>(u, Fl, R(ph.gamma));
     }
-    if constexpr (VISC) load_G();
 #pragma unroll
     for (int d1 = 0; d1 < D; d1++)
 #pragma unroll
@@ -454,6 +455,25 @@ __global__ __launch_bounds__(kMaxBlock) void k_volume(int E, const double* __res
       disFne[(ee * N + k) * F + f] = a;
     }
   }
+}
+
+// The volume kernel, and the same kernel held to 4 waves per SIMD (at most
+// 128 VGPRs) for the shapes that fit there without spilling: its LDS slabs
+// limit it to 4 waves per SIMD anyway, and the cap lets the metric terms stay
+// in registers across the whole element.
+template <int D, int P, bool VISC, class R>
+__global__ __launch_bounds__(kMaxBlock) void k_volume(int E, const double* __restrict__ Ue, const R* __restrict__ dUce,
+                                                 const double* __restrict__ JGe, const double* __restrict__ detJe,
+                                                 R* __restrict__ divFe, R* __restrict__ disFne, R* __restrict__ dUfe,
+                                                 Tabs tb, Phys ph) {
+  volume_body<D, P, VISC, R>(E, Ue, dUce, JGe, detJe, divFe, disFne, dUfe, tb, ph);
+}
+template <int D, int P, bool VISC, class R>
+__global__ __launch_bounds__(kMaxBlock) __attribute__((amdgpu_waves_per_eu(4))) void k_volume_w4(
+    int E, const double* __restrict__ Ue, const R* __restrict__ dUce, const double* __restrict__ JGe,
+    const double* __restrict__ detJe, R* __restrict__ divFe, R* __restrict__ disFne, R* __restrict__ dUfe, Tabs tb,
+    Phys ph) {
+  volume_body<D, P, VISC, R>(E, Ue, dUce, JGe, detJe, divFe, disFne, dUfe, tb, ph);
 }
 
 // ---------------------------------------------------------------------------
@@ -1036,17 +1056,22 @@ struct Runner {
                        w.Ufe, st.norme, st.dAe, w.Fne, w.FnInv, st.waveSpe, w.dUce, c.ph, st.bst);
     FCHK(hipGetLastError());
   }
-  static void launch_volume(Ctx& c, int nt) {
+  template <bool VISC>
+  static void launch_volume_t(Ctx& c, int nt) {
     FusedState& st = fs(c);
     Work<R>& w = st.work<R>();
-    const bool visc = c.prm.viscous != 0;
-    if (visc)
-      hipLaunchKernelGGL((k_volume<D, P, true, R>), dim3(nblk(c.z.E, nt / S)), dim3(nt), lds(kVolume, nt, true),
+    // FP64 hexes above P3 need more than 128 VGPRs: no cap there
+    if constexpr (D == 2 || P <= 3 || sizeof(R) == 4)
+      hipLaunchKernelGGL((k_volume_w4<D, P, VISC, R>), dim3(nblk(c.z.E, nt / S)), dim3(nt), lds(kVolume, nt, VISC),
                          c.stream, c.z.E, st.Ue, w.dUce, st.JGe, st.detJe, w.divFe, w.disFne, w.dUfe, st.tb, c.ph);
     else
-      hipLaunchKernelGGL((k_volume<D, P, false, R>), dim3(nblk(c.z.E, nt / S)), dim3(nt), lds(kVolume, nt, false),
+      hipLaunchKernelGGL((k_volume<D, P, VISC, R>), dim3(nblk(c.z.E, nt / S)), dim3(nt), lds(kVolume, nt, VISC),
                          c.stream, c.z.E, st.Ue, w.dUce, st.JGe, st.detJe, w.divFe, w.disFne, w.dUfe, st.tb, c.ph);
     FCHK(hipGetLastError());
+  }
+  static void launch_volume(Ctx& c, int nt) {
+    if (c.prm.viscous) launch_volume_t<true>(c, nt);
+    else launch_volume_t<false>(c, nt);
   }
   static void launch_face_visc(Ctx& c, int nt) {
     FusedState& st = fs(c);
