@@ -287,7 +287,7 @@ def main() -> int:
                                f"hex, P{a.order}, Rusanov{'+LDG' if visc else ''}, RK44, FP64",
                    "elements": setup.nEles, "dof": dof, "stages_per_step": nRK, "path": a.path,
                    "dt": setup.params.dt,
-                   "block_sizes": tuned or "default 256 (extrapolate, face_inviscid, volume, face_viscous, rk)"},
+                   "block_sizes": tuned or "default 256 (extrapolate, face_inviscid, volume, face_boundary, rk)"},
         "roofline": roof,
         "cpu_baseline": cpu,
         "kernels": kernels,

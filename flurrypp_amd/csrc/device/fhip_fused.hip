@@ -63,7 +63,7 @@ struct Shape {
 extern __shared__ __align__(16) unsigned char fhip_lds[];
 constexpr int kMaxBlock = 512;
 constexpr size_t kMaxLds = 64 * 1024;
-enum { kExtrap = 0, kFaceInv = 1, kVolume = 2, kFaceVisc = 3, kRk = 4 };
+enum { kExtrap = 0, kFaceInv = 1, kVolume = 2, kFaceBnd = 3, kRk = 4 };
 
 // Sparse operator tables (device pointers into one allocation).  Indices of
 // solution points are positions in the swizzled LDS layout (spt_lds below).
@@ -107,12 +107,22 @@ __host__ __device__ constexpr int spt_lds(int D, int P1, int s) {
 // Face and volume intermediates of one precision.
 template <class R>
 struct Work {
-  R *divFe = nullptr, *Ufe = nullptr, *Fne = nullptr, *disFne = nullptr, *dUce = nullptr, *dUfe = nullptr,
-    *FnInv = nullptr;
+  R *divFe = nullptr, *Ufe = nullptr, *Fne = nullptr, *disFne = nullptr, *dUce = nullptr;
+  // viscous runs: each flux point's side of the LDG flux [ele][N][F], and the
+  // gradient at the boundary flux points [D*N][nBnd]
+  R *Bve = nullptr, *gBnd = nullptr;
   R* Rst = nullptr;  // mixed path: residual of each classical RK stage but the last
+  // mixed path: U at the solution points rounded to FP32, the only form in
+  // which the flux kernels read it; the RK stages inside a step keep their
+  // states here and write the FP64 state only at the step's last stage
+  R* Us = nullptr;
+  // geometry in the kernels' arithmetic type: the FP64 arrays themselves, or
+  // FP32 copies rounded once (the values the FP32 kernels would convert to)
+  const R *JG = nullptr, *iJ = nullptr, *nrm = nullptr, *dA = nullptr, *nf = nullptr;
+  void* geo_mem = nullptr;
   bool ready = false;
   void release() {
-    void* ps[] = {divFe, Ufe, Fne, disFne, dUce, dUfe, FnInv, Rst};
+    void* ps[] = {divFe, Ufe, Fne, disFne, dUce, Bve, gBnd, Rst, Us, geo_mem};
     for (void* p : ps)
       if (p) (void)hipFree(p);
     *this = Work<R>();
@@ -124,8 +134,14 @@ struct FusedState {
   void* tab_mem = nullptr;
   // FP64 element-major state, metrics and wave speeds (both paths)
   double *Ue = nullptr, *U0e = nullptr, *Uacc = nullptr, *divF0e = nullptr, *JGe = nullptr, *detJe = nullptr,
-         *norme = nullptr, *dAe = nullptr, *waveSpe = nullptr;
+         *iJe = nullptr, *norme = nullptr, *dAe = nullptr, *waveSpe = nullptr;
   int *pL = nullptr, *pR = nullptr, *bc = nullptr;
+  // viscous runs: per element flux point, 4 * (partner flux point e*F + f) +
+  // side (0 left, 1 right) or 4 * boundary slot + 2; the face's normal (the
+  // left element's) at every element flux point; boundary face points by slot
+  int *fpart = nullptr, *bndT = nullptr;
+  double* nfe = nullptr;
+  int nBnd = 0;
   // batching: boundary-state table, element -> mesh map, per-mesh dt and activity
   int nMesh = 1;
   std::vector<int> h_meshStart;
@@ -141,8 +157,8 @@ struct FusedState {
   ~FusedState() {
     w64.release();
     w32.release();
-    void* ps[] = {tab_mem, Ue,  U0e,    Uacc,   divF0e,    JGe,    detJe, norme, dAe, waveSpe,
-                  pL,      pR,  bc,     bst,    meshDt,    meshRes, eleMesh, meshStart, active, srce};
+    void* ps[] = {tab_mem, Ue,  U0e,     Uacc,    divF0e,    JGe,    detJe,  iJe,   norme, dAe, waveSpe, pL,
+                  pR,      bc,  fpart,   bndT,    nfe,       bst,    meshDt, meshRes, eleMesh, meshStart, active, srce};
     for (void* p : ps)
       if (p) (void)hipFree(p);
   }
@@ -181,7 +197,7 @@ __global__ void k_from_em(long long total, int P, int E, int K, const double* __
 // U at the flux points from U at the solution points (opp_spts_to_fpts rows).
 template <int D, int P, class R>
 __global__ __launch_bounds__(kMaxBlock) void k_extrap(int E, const double* __restrict__ Ue, R* __restrict__ Ufe,
-                                                       Tabs tb) {
+                                                       R* __restrict__ Us, Tabs tb) {
   using Sh = Shape<D, P>;
   constexpr int S = Sh::S, F = Sh::F, N = Sh::N, P1 = Sh::P1;
   const int EB = blockDim.x / S;
@@ -190,7 +206,9 @@ __global__ __launch_bounds__(kMaxBlock) void k_extrap(int E, const double* __res
   const double* src = Ue + (long long)e0 * N * S;
   for (int i = threadIdx.x; i < nb * N * S; i += blockDim.x) {
     const int r = i / S, q = i - r * S;
-    sU[r * S + spt_lds(D, P1, q)] = R(src[i]);
+    const R v = R(src[i]);
+    sU[r * S + spt_lds(D, P1, q)] = v;
+    if (Us) Us[(long long)e0 * N * S + i] = v;
   }
   __syncthreads();
   for (int i = threadIdx.x; i < nb * F; i += blockDim.x) {
@@ -229,9 +247,9 @@ __device__ __forceinline__ void bc_state(int code, const R* UL, const R* n, R* U
 template <int D, int P, class R>
 __global__ __launch_bounds__(kMaxBlock) void k_face_inv(int nFF, const int* __restrict__ pLa, const int* __restrict__ pRa,
                                                    const int* __restrict__ bca, const R* __restrict__ Ufe,
-                                                   const double* __restrict__ norme, const double* __restrict__ dAe,
-                                                   R* __restrict__ Fne, R* __restrict__ FnInv,
-                                                   double* __restrict__ waveSpe, R* __restrict__ dUce, Phys ph,
+                                                   const R* __restrict__ norme, const R* __restrict__ dAe,
+                                                   R* __restrict__ Fne, double* __restrict__ waveSpe,
+                                                   R* __restrict__ dUce, Phys ph,
                                                    const double* __restrict__ bst) {
   using Sh = Shape<D, P>;
   constexpr int F = Sh::F, N = Sh::N;
@@ -244,7 +262,7 @@ __global__ __launch_bounds__(kMaxBlock) void k_face_inv(int nFF, const int* __re
 #pragma unroll
   for (int k = 0; k < N; k++) UL[k] = Ufe[bL + k * F];
 #pragma unroll
-  for (int d = 0; d < D; d++) n[d] = R(norme[(eL * D + d) * F + fL]);
+  for (int d = 0; d < D; d++) n[d] = norme[(eL * D + d) * F + fL];
   long long bR = 0;
   if (pr < 0) {
     bc_state<D, R>(bc, UL, n, UR, ph, bst);
@@ -258,9 +276,17 @@ __global__ __launch_bounds__(kMaxBlock) void k_face_inv(int nFF, const int* __re
   else {
     if constexpr (D == 2) roe2d<R>(UL, UR, n, fn, R(ph.gamma));
   }
-  if (ph.viscous) {
+  // the inviscid common flux times dA on both sides; viscous runs add the
+  // LDG flux in the RK kernel, where both sides' parts meet
+  const R dAL = dAe[pl];
 #pragma unroll
-    for (int k = 0; k < N; k++) FnInv[(long long)k * nFF + t] = fn[k];
+  for (int k = 0; k < N; k++) Fne[bL + k * F] = fn[k] * dAL;
+  if (pr >= 0) {
+    const R dAR = dAe[pr];
+#pragma unroll
+    for (int k = 0; k < N; k++) Fne[bR + k * F] = -fn[k] * dAR;
+  }
+  if (ph.viscous) {
     R pf = (pr < 0) ? R(0) : ldg_pen<D, R>(n, R(ph.pen_fact));
 #pragma unroll
     for (int k = 0; k < N; k++) {
@@ -268,26 +294,29 @@ __global__ __launch_bounds__(kMaxBlock) void k_face_inv(int nFF, const int* __re
       dUce[bL + k * F] = uc - UL[k];
       if (pr >= 0) dUce[bR + k * F] = uc - UR[k];
     }
-  } else {
-    const R dAL = R(dAe[pl]);
-#pragma unroll
-    for (int k = 0; k < N; k++) Fne[bL + k * F] = fn[k] * dAL;
-    if (pr >= 0) {
-      const R dAR = R(dAe[pr]);
-#pragma unroll
-      for (int k = 0; k < N; k++) Fne[bR + k * F] = -fn[k] * dAR;
-    }
   }
 }
+
+// Viscous runs: what the volume kernel leaves at the flux points in place of
+// the gradient there (see the face loop of volume_body).
+template <class R>
+struct VFace {
+  const int* fpart;    // [E*F] 4 * partner flux point + side, or 4 * boundary slot + 2
+  const R* nfe;        // [E][D][F] the face's normal (its left element's)
+  R* Bve;              // [E][N][F] this side's part of the LDG flux . n
+  R* gBnd;             // [D*N][nBnd] gradient at the boundary flux points
+  int nBnd;
+};
 
 // ---------------------------------------------------------------------------
 // Volume kernel: [corrected gradient], flux, divergence, normal flux at faces
 // [and gradient at faces].  One thread per solution point, EB elements per
 // block, element data staged in LDS.
 template <int D, int P, bool VISC, class R>
-__device__ __forceinline__ void volume_body(int E, const double* __restrict__ Ue, const R* __restrict__ dUce,
-                                            const double* __restrict__ JGe, const double* __restrict__ detJe,
-                                            R* __restrict__ divFe, R* __restrict__ disFne, R* __restrict__ dUfe,
+__device__ __forceinline__ void volume_body(int E, const double* __restrict__ Ue, const R* __restrict__ Us,
+                                            const R* __restrict__ dUce,
+                                            const R* __restrict__ JGe, const R* __restrict__ iJe,
+                                            R* __restrict__ divFe, R* __restrict__ disFne, VFace<R> vf,
                                             Tabs tb, Phys ph) {
   using Sh = Shape<D, P>;
   constexpr int S = Sh::S, F = Sh::F, N = Sh::N, P1 = Sh::P1, NC = Sh::NC;
@@ -310,16 +339,24 @@ __device__ __forceinline__ void volume_body(int E, const double* __restrict__ Ue
 #pragma unroll
     for (int r = 0; r < D; r++)
 #pragma unroll
-      for (int x = 0; x < D; x++) G[r][x] = R(JGe[((e * D + r) * D + x) * S + s]);
+      for (int x = 0; x < D; x++) G[r][x] = JGe[((e * D + r) * D + x) * S + s];
   };
   if constexpr (!VISC) {
     if (act) load_G();
   }
   {
-    const double* src = Ue + (long long)e0 * N * S;
-    for (int i = threadIdx.x; i < nb * N * S; i += blockDim.x) {
-      const int r = i / S, q = i - r * S;
-      sU[r * S + spt_lds(D, P1, q)] = R(src[i]);
+    if (Us) {
+      const R* src = Us + (long long)e0 * N * S;
+      for (int i = threadIdx.x; i < nb * N * S; i += blockDim.x) {
+        const int r = i / S, q = i - r * S;
+        sU[r * S + spt_lds(D, P1, q)] = src[i];
+      }
+    } else {
+      const double* src = Ue + (long long)e0 * N * S;
+      for (int i = threadIdx.x; i < nb * N * S; i += blockDim.x) {
+        const int r = i / S, q = i - r * S;
+        sU[r * S + spt_lds(D, P1, q)] = R(src[i]);
+      }
     }
     if constexpr (VISC) {
       // dUc at the block's flux points (sF serves as sC until the gradient is written)
@@ -338,7 +375,7 @@ __device__ __forceinline__ void volume_body(int E, const double* __restrict__ Ue
     if (act) {
       // metric loads in flight while the reference gradient is formed
       load_G();
-      inv = R(1. / detJe[e * S + s]);
+      inv = iJe[e * S + s];
 #pragma unroll
       for (int d = 0; d < D; d++) {
         const int* gi = tb.grd_idx + (d * S + s) * P1;
@@ -370,23 +407,55 @@ __device__ __forceinline__ void volume_body(int E, const double* __restrict__ Ue
         }
     }
     __syncthreads();
-    // physical gradient at the flux points (extrapolateGradU)
+    // physical gradient at the flux points (extrapolateGradU) and, with U
+    // there (extrapolateU, the same sums as the RK kernel's), this side's
+    // viscous flux folded into its part of the LDG flux
+    // (face::calcViscousFlux, src/face.cpp:216-293); the two sides' parts are
+    // summed where the correction is applied.  Boundary points keep the
+    // gradient for the boundary kernel, which needs the boundary state.
     for (int i = threadIdx.x; i < nb * F; i += blockDim.x) {
       const int bb = i / F, f = i - bb * F;
       const long long ee = e0 + bb;
       const int* xi = tb.ext_idx + f * P1;
       const R* xw = tw<R>(tb.ext_w, tb.ext_wf) + f * P1;
-#pragma unroll 1
+      const int code = vf.fpart[ee * F + f], side = code & 3;
+      R g[D][N];
+#pragma unroll
       for (int x = 0; x < D; x++)
 #pragma unroll
         for (int k = 0; k < N; k++) {
           R a = 0;
 #pragma unroll
           for (int m = 0; m < P1; m++) a += xw[m] * sG[((bb * D + x) * N + k) * S + xi[m]];
-          dUfe[((ee * D + x) * N + k) * F + f] = a;
+          g[x][k] = a;
         }
+      if (side == 2) {
+        const long long slot = code >> 2;
+#pragma unroll
+        for (int x = 0; x < D; x++)
+#pragma unroll
+          for (int k = 0; k < N; k++) vf.gBnd[(x * N + k) * (long long)vf.nBnd + slot] = g[x][k];
+      } else {
+        R u[N], nv[D], bv[N];
+#pragma unroll
+        for (int k = 0; k < N; k++) {
+          R a = 0;
+#pragma unroll
+          for (int m = 0; m < P1; m++) a += xw[m] * sU[(bb * N + k) * S + xi[m]];
+          u[k] = a;
+          bv[k] = 0;
+        }
+#pragma unroll
+        for (int x = 0; x < D; x++) nv[x] = vf.nfe[(ee * D + x) * F + f];
+        ldg_add_side<D, R>(side == 0, u, g, nv, bv, ph);
+#pragma unroll
+        for (int k = 0; k < N; k++) vf.Bve[(ee * N + k) * F + f] = bv[k];
+      }
     }
     __syncthreads();
+    // the metric terms again (from cache): keeping them live across the face
+    // loop above costs more registers than the hold allows
+    if (act) load_G();
   }
   if (act) {
     // phase 2: flux at the point, transformed to the reference element
@@ -462,76 +531,58 @@ __device__ __forceinline__ void volume_body(int E, const double* __restrict__ Ue
 // limit it to 4 waves per SIMD anyway, and the cap lets the metric terms stay
 // in registers across the whole element.
 template <int D, int P, bool VISC, class R>
-__global__ __launch_bounds__(kMaxBlock) void k_volume(int E, const double* __restrict__ Ue, const R* __restrict__ dUce,
-                                                 const double* __restrict__ JGe, const double* __restrict__ detJe,
-                                                 R* __restrict__ divFe, R* __restrict__ disFne, R* __restrict__ dUfe,
+__global__ __launch_bounds__(kMaxBlock) void k_volume(int E, const double* __restrict__ Ue, const R* __restrict__ Us,
+                                                 const R* __restrict__ dUce,
+                                                 const R* __restrict__ JGe, const R* __restrict__ iJe,
+                                                 R* __restrict__ divFe, R* __restrict__ disFne, VFace<R> vf,
                                                  Tabs tb, Phys ph) {
-  volume_body<D, P, VISC, R>(E, Ue, dUce, JGe, detJe, divFe, disFne, dUfe, tb, ph);
+  volume_body<D, P, VISC, R>(E, Ue, Us, dUce, JGe, iJe, divFe, disFne, vf, tb, ph);
 }
 template <int D, int P, bool VISC, class R>
 __global__ __launch_bounds__(kMaxBlock) __attribute__((amdgpu_waves_per_eu(4))) void k_volume_w4(
-    int E, const double* __restrict__ Ue, const R* __restrict__ dUce, const double* __restrict__ JGe,
-    const double* __restrict__ detJe, R* __restrict__ divFe, R* __restrict__ disFne, R* __restrict__ dUfe, Tabs tb,
+    int E, const double* __restrict__ Ue, const R* __restrict__ Us, const R* __restrict__ dUce, const R* __restrict__ JGe,
+    const R* __restrict__ iJe, R* __restrict__ divFe, R* __restrict__ disFne, VFace<R> vf, Tabs tb,
     Phys ph) {
-  volume_body<D, P, VISC, R>(E, Ue, dUce, JGe, detJe, divFe, disFne, dUfe, tb, ph);
+  volume_body<D, P, VISC, R>(E, Ue, Us, dUce, JGe, iJe, divFe, disFne, vf, tb, ph);
 }
 
 // ---------------------------------------------------------------------------
-// LDG common viscous flux at face flux points.
+// LDG flux at the boundary flux points (face::calcViscousFlux with the
+// boundary state of boundFace::applyViscousBCs, src/face.cpp:216-293,
+// src/boundFace.cpp:316-352): one thread per boundary point, from the
+// gradient the volume kernel left there; the result goes where the volume
+// kernel puts an interior point's side of the flux.
 template <int D, int P, class R>
-__global__ __launch_bounds__(kMaxBlock) void k_face_visc(int nFF, const int* __restrict__ pLa, const int* __restrict__ pRa,
-                                                    const int* __restrict__ bca, const R* __restrict__ Ufe,
-                                                    const R* __restrict__ dUfe, const double* __restrict__ norme,
-                                                    const double* __restrict__ dAe, R* __restrict__ Fne,
-                                                    const R* __restrict__ FnInv, Phys ph,
-                                                    const double* __restrict__ bst) {
+__global__ __launch_bounds__(kMaxBlock) void k_face_bnd(int nB, const int* __restrict__ bndT,
+                                                   const int* __restrict__ pLa, const int* __restrict__ bca,
+                                                   const R* __restrict__ Ufe, const R* __restrict__ gBnd,
+                                                   const R* __restrict__ norme, R* __restrict__ Bve, Phys ph,
+                                                   const double* __restrict__ bst) {
   using Sh = Shape<D, P>;
   constexpr int F = Sh::F, N = Sh::N;
-  int t = blockIdx.x * blockDim.x + threadIdx.x;
-  if (t >= nFF) return;
-  const int pl = pLa[t], pr = pRa[t], bc = bca[t];
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= nB) return;
+  const int t = bndT[i], pl = pLa[t], bc = bca[t];
   const long long eL = pl / F, fL = pl - eL * F;
   const long long bL = eL * N * F + fL;
-  R UL[N], UR[N], n[D], gL[D][N], gR[D][N], fn[N];
+  R UL[N], UR[N], n[D], g[D][N], nF[N], d12[N], bv[N];
 #pragma unroll
   for (int k = 0; k < N; k++) UL[k] = Ufe[bL + k * F];
 #pragma unroll
-  for (int d = 0; d < D; d++) n[d] = R(norme[(eL * D + d) * F + fL]);
+  for (int d = 0; d < D; d++) n[d] = norme[(eL * D + d) * F + fL];
 #pragma unroll
   for (int d = 0; d < D; d++)
 #pragma unroll
-    for (int k = 0; k < N; k++) gL[d][k] = dUfe[((eL * D + d) * N + k) * F + fL];
-  int bnd = 0;
-  long long bR = 0;
-  if (pr < 0) {
-    bc_state<D, R>(bc, UL, n, UR, ph, bst);
-    bnd = ((bc & 15) == 10) ? 2 : 1;
+    for (int k = 0; k < N; k++) g[d][k] = gBnd[(d * N + k) * (long long)nB + i];
+  bc_state<D, R>(bc, UL, n, UR, ph, bst);
+  const bool adiabatic = (bc & 15) == 10;
+  if (adiabatic) adiabatic_grad<D, R>(UL, UR, g, R(ph.gamma));
+  ldg_side<D, R>(adiabatic ? UR : UL, g, n, nF, d12, ph);
 #pragma unroll
-    for (int d = 0; d < D; d++)
+  for (int k = 0; k < N; k++) bv[k] = 0;
+  ldg_boundary<D, R>(UL, UR, nF, n, bv, ph);
 #pragma unroll
-      for (int k = 0; k < N; k++) gR[d][k] = gL[d][k];
-    if (bnd == 2) adiabatic_grad<D, R>(UL, UR, gR, R(ph.gamma));
-  } else {
-    const long long eR = pr / F, fR = pr - eR * F;
-    bR = eR * N * F + fR;
-#pragma unroll
-    for (int k = 0; k < N; k++) UR[k] = Ufe[bR + k * F];
-#pragma unroll
-    for (int d = 0; d < D; d++)
-#pragma unroll
-      for (int k = 0; k < N; k++) gR[d][k] = dUfe[((eR * D + d) * N + k) * F + fR];
-  }
-#pragma unroll
-  for (int k = 0; k < N; k++) fn[k] = FnInv[(long long)k * nFF + t];
-  ldg_flux<D, R>(bnd, UL, UR, gL, gR, n, fn, ph);
-  const R dAL = R(dAe[pl]);
-#pragma unroll
-  for (int k = 0; k < N; k++) Fne[bL + k * F] = fn[k] * dAL;
-  if (pr >= 0) {
-    const R dAR = R(dAe[pr]);
-#pragma unroll
-    for (int k = 0; k < N; k++) Fne[bR + k * F] = -fn[k] * dAR;
-  }
+  for (int k = 0; k < N; k++) Bve[bL + k * F] = bv[k];
 }
 
 // ---------------------------------------------------------------------------
@@ -555,7 +606,9 @@ __global__ __launch_bounds__(kMaxBlock) void k_rk(int E, int mode, double a, dou
                                              double* __restrict__ divF0e, R* __restrict__ Ufe, Tabs tb,
                                              const int* __restrict__ eleMesh, const double* __restrict__ meshDt,
                                              const int* __restrict__ active, const double* __restrict__ srce,
-                                             R* __restrict__ Rst, int stage, RkB bs) {
+                                             R* __restrict__ Rst, int stage, RkB bs, const int* __restrict__ fpart,
+                                             const R* __restrict__ dAe, const R* __restrict__ Bve, R* __restrict__ Us,
+                                             int fin) {
   using Sh = Shape<D, P>;
   constexpr int S = Sh::S, F = Sh::F, N = Sh::N, P1 = Sh::P1, NC = Sh::NC;
   const int EB = blockDim.x / S;
@@ -564,7 +617,28 @@ __global__ __launch_bounds__(kMaxBlock) void k_rk(int E, int mode, double a, dou
   R* const sD = sU + EB * N * S;                  // [EB*N*F] Fn - disFn at the block's flux points
   const int e0 = blockIdx.x * EB, nb = min(EB, E - e0);
   const int b = threadIdx.x / S, s = threadIdx.x - b * S;
-  {
+  if (Bve) {
+    // viscous: common flux = inviscid part (Fne, already times dA) plus both
+    // sides' LDG parts times dA, negated on the right side
+    for (int i = threadIdx.x; i < nb * F; i += blockDim.x) {
+      const int bb = i / F, f = i - bb * F;
+      const long long ef = (long long)(e0 + bb) * F + f, ob = (long long)(e0 + bb) * N * F + f;
+      const int code = fpart[ef], side = code & 3;
+      const R dA = dAe[ef];
+      long long pb = ob;
+      if (side != 2) {
+        const long long p = code >> 2, ep = p / F;
+        pb = ep * N * F + (p - ep * F);
+      }
+#pragma unroll
+      for (int k = 0; k < N; k++) {
+        R bv = Bve[ob + k * F];
+        if (side != 2) bv += Bve[pb + k * F];
+        if (side == 1) bv = -bv;
+        sD[(bb * N + k) * F + f] = (Fne[ob + k * F] + bv * dA) - disFne[ob + k * F];
+      }
+    }
+  } else {
     const long long o = (long long)e0 * N * F;
     for (int i = threadIdx.x; i < nb * N * F; i += blockDim.x) sD[i] = Fne[o + i] - disFne[o + i];
   }
@@ -599,7 +673,9 @@ __global__ __launch_bounds__(kMaxBlock) void k_rk(int E, int mode, double a, dou
       if (divF0e) divF0e[iu] = r0;
       // P-multigrid source: divF_spts[step] + src_spts (src/solver.cpp:480,514)
       const double r = srce ? r0 + srce[iu] : r0;
-      const double u = Ue[iu];
+      // U itself is an input of the first and the single stage only (and of
+      // frozen batch members, which keep it)
+      const double u = (mode == 0 || mode == 3 || frozen) ? Ue[iu] : 0.0;
       double un;
       if (mode == 0) {
         U0e[iu] = u;
@@ -625,7 +701,10 @@ __global__ __launch_bounds__(kMaxBlock) void k_rk(int E, int mode, double a, dou
         un = U0e[iu] - a * r / dj * dt;
       }
       if (frozen) un = u;
-      Ue[iu] = un;
+      // with an FP32 stage copy, the FP64 state is written at the step's
+      // last stage only (earlier stages leave it at the step's start)
+      if (!Us || fin) Ue[iu] = un;
+      if (Us) Us[iu] = R(un);
       sU[(b * N + k) * S + spt_lds(D, P1, s)] = R(un);
     }
   }
@@ -862,6 +941,40 @@ static bool build_tables(const Ctx& c, const fhip_desc* d, std::vector<char>& bl
   return true;
 }
 
+__global__ void k_round_f32(long long n, const double* __restrict__ in, float* __restrict__ out) {
+  for (long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (long long)gridDim.x * blockDim.x)
+    out[i] = float(in[i]);
+}
+
+// Geometry views of one precision (Work::JG .. nf).
+static void set_geometry(Ctx& c, Work<double>& w) {
+  FusedState& st = fs(c);
+  w.JG = st.JGe;
+  w.iJ = st.iJe;
+  w.nrm = st.norme;
+  w.dA = st.dAe;
+  w.nf = st.nfe;
+}
+static void set_geometry(Ctx& c, Work<float>& w) {
+  FusedState& st = fs(c);
+  const long long E = c.z.E, S = c.z.S, F = c.z.F, D = c.z.D;
+  const long long n[5] = {E * D * D * S, E * S, E * D * F, E * F, st.nfe ? E * D * F : 0};
+  const double* src[5] = {st.JGe, st.iJe, st.norme, st.dAe, st.nfe};
+  long long tot = 0;
+  for (long long v : n) tot += (v + 63) / 64 * 64;
+  float* m = dal<float>(tot);
+  w.geo_mem = m;
+  const float** dst[5] = {&w.JG, &w.iJ, &w.nrm, &w.dA, &w.nf};
+  for (int i = 0; i < 5; i++) {
+    *dst[i] = n[i] ? m : nullptr;
+    if (n[i]) {
+      hipLaunchKernelGGL(k_round_f32, dim3(1024), dim3(256), 0, c.stream, n[i], src[i], m);
+      FCHK(hipGetLastError());
+    }
+    m += (n[i] + 63) / 64 * 64;
+  }
+}
+
 template <class R>
 static void ensure_work(Ctx& c) {
   Work<R>& w = fs(c).work<R>();
@@ -873,10 +986,12 @@ static void ensure_work(Ctx& c) {
   w.disFne = dal<R>(E * N * F);
   if (c.prm.viscous) {
     w.dUce = dal<R>(E * N * F);
-    w.dUfe = dal<R>(E * D * N * F);
-    w.FnInv = dal<R>((long long)c.z.nFaceFpts * N);
+    w.Bve = dal<R>(E * N * F);
+    w.gBnd = dal<R>((long long)D * N * fs(c).nBnd);
   }
   if (sizeof(R) == 4 && c.z.nRK > 1 && c.prm.time_type < 5) w.Rst = dal<R>((long long)(c.z.nRK - 1) * E * N * S);
+  if (sizeof(R) == 4) w.Us = dal<R>(E * N * S);
+  set_geometry(c, w);
   w.ready = true;
 }
 
@@ -904,6 +1019,7 @@ bool fused_setup(Ctx& c, const fhip_desc* d) {
   st->waveSpe = dal<double>(E * F);
   st->JGe = dal<double>(E * D * D * S);
   st->detJe = dal<double>(E * S);
+  st->iJe = dal<double>(E * S);
   st->norme = dal<double>(E * D * F);
   st->dAe = dal<double>(E * F);
   {
@@ -922,6 +1038,8 @@ bool fused_setup(Ctx& c, const fhip_desc* d) {
       }
     FCHK(hipMemcpy(st->JGe, jg.data(), jg.size() * 8, hipMemcpyHostToDevice));
     FCHK(hipMemcpy(st->detJe, dj.data(), dj.size() * 8, hipMemcpyHostToDevice));
+    for (double& v : dj) v = 1. / v;
+    FCHK(hipMemcpy(st->iJe, dj.data(), dj.size() * 8, hipMemcpyHostToDevice));
     FCHK(hipMemcpy(st->norme, nr.data(), nr.size() * 8, hipMemcpyHostToDevice));
     FCHK(hipMemcpy(st->dAe, da.data(), da.size() * 8, hipMemcpyHostToDevice));
   }
@@ -971,6 +1089,38 @@ bool fused_setup(Ctx& c, const fhip_desc* d) {
     FCHK(hipMemcpy(st->pL, pl.data(), pl.size() * 4, hipMemcpyHostToDevice));
     FCHK(hipMemcpy(st->pR, pr.data(), pr.size() * 4, hipMemcpyHostToDevice));
     FCHK(hipMemcpy(st->bc, bc.data(), bc.size() * 4, hipMemcpyHostToDevice));
+    if (c.prm.viscous) {
+      // every element flux point must lie on exactly one face point pair
+      if (E * F >= (1LL << 29)) throw std::invalid_argument("fused path: more than 2^29 element flux points");
+      std::vector<int> part((size_t)E * F, -1), bt;
+      std::vector<double> nl((size_t)E * D * F);
+      // the left element's normal at flux point q (element-major q = e*F + f)
+      auto put_normal = [&](int q, int qL) {
+        const long long e = q / F, f = q - e * F, eL = qL / F, fL = qL - eL * F;
+        for (long long x = 0; x < D; x++) nl[(e * D + x) * F + f] = d->norm_fpts[(fL * E + eL) * D + x];
+      };
+      for (int t = 0; t < z.nFaceFpts; t++) {
+        if (part[pl[t]] != -1 || (pr[t] >= 0 && part[pr[t]] != -1)) return false;
+        put_normal(pl[t], pl[t]);
+        if (pr[t] < 0) {
+          bt.push_back(t);
+          part[pl[t]] = 4 * (int)(bt.size() - 1) + 2;
+        } else {
+          part[pl[t]] = 4 * pr[t];
+          part[pr[t]] = 4 * pl[t] + 1;
+          put_normal(pr[t], pl[t]);
+        }
+      }
+      for (int q : part)
+        if (q == -1) return false;
+      st->nBnd = (int)bt.size();
+      st->fpart = dal<int>(part.size());
+      st->bndT = dal<int>(bt.size());
+      st->nfe = dal<double>(nl.size());
+      FCHK(hipMemcpy(st->fpart, part.data(), part.size() * 4, hipMemcpyHostToDevice));
+      FCHK(hipMemcpy(st->nfe, nl.data(), nl.size() * 8, hipMemcpyHostToDevice));
+      if (!bt.empty()) FCHK(hipMemcpy(st->bndT, bt.data(), bt.size() * 4, hipMemcpyHostToDevice));
+    }
   }
   return true;
 }
@@ -1037,14 +1187,14 @@ struct Runner {
   }
   static bool valid(int k, int nt, bool visc) {
     if (nt < 64 || nt > kMaxBlock) return false;
-    if (k == kFaceInv || k == kFaceVisc) return true;
+    if (k == kFaceInv || k == kFaceBnd) return true;
     return nt >= S && lds(k, nt, visc) <= kMaxLds;
   }
   static void launch_extrap(Ctx& c, int nt) {
     FusedState& st = fs(c);
     Work<R>& w = st.work<R>();
     hipLaunchKernelGGL((k_extrap<D, P, R>), dim3(nblk(c.z.E, nt / S)), dim3(nt), lds(kExtrap, nt, false), c.stream,
-                       c.z.E, st.Ue, w.Ufe, st.tb);
+                       c.z.E, st.Ue, w.Ufe, w.Us, st.tb);
     FCHK(hipGetLastError());
   }
   static void launch_face_inv(Ctx& c, int nt) {
@@ -1053,33 +1203,33 @@ struct Runner {
     const int nFF = c.z.nFaceFpts;
     if (nFF == 0) return;
     hipLaunchKernelGGL((k_face_inv<D, P, R>), dim3(nblk(nFF, nt)), dim3(nt), 0, c.stream, nFF, st.pL, st.pR, st.bc,
-                       w.Ufe, st.norme, st.dAe, w.Fne, w.FnInv, st.waveSpe, w.dUce, c.ph, st.bst);
+                       w.Ufe, w.nrm, w.dA, w.Fne, st.waveSpe, w.dUce, c.ph, st.bst);
     FCHK(hipGetLastError());
   }
   template <bool VISC>
   static void launch_volume_t(Ctx& c, int nt) {
     FusedState& st = fs(c);
     Work<R>& w = st.work<R>();
+    const VFace<R> vf{st.fpart, w.nf, w.Bve, w.gBnd, st.nBnd};
     // FP64 hexes above P3 need more than 128 VGPRs: no cap there
     if constexpr (D == 2 || P <= 3 || sizeof(R) == 4)
       hipLaunchKernelGGL((k_volume_w4<D, P, VISC, R>), dim3(nblk(c.z.E, nt / S)), dim3(nt), lds(kVolume, nt, VISC),
-                         c.stream, c.z.E, st.Ue, w.dUce, st.JGe, st.detJe, w.divFe, w.disFne, w.dUfe, st.tb, c.ph);
+                         c.stream, c.z.E, st.Ue, w.Us, w.dUce, w.JG, w.iJ, w.divFe, w.disFne, vf, st.tb, c.ph);
     else
       hipLaunchKernelGGL((k_volume<D, P, VISC, R>), dim3(nblk(c.z.E, nt / S)), dim3(nt), lds(kVolume, nt, VISC),
-                         c.stream, c.z.E, st.Ue, w.dUce, st.JGe, st.detJe, w.divFe, w.disFne, w.dUfe, st.tb, c.ph);
+                         c.stream, c.z.E, st.Ue, w.Us, w.dUce, w.JG, w.iJ, w.divFe, w.disFne, vf, st.tb, c.ph);
     FCHK(hipGetLastError());
   }
   static void launch_volume(Ctx& c, int nt) {
     if (c.prm.viscous) launch_volume_t<true>(c, nt);
     else launch_volume_t<false>(c, nt);
   }
-  static void launch_face_visc(Ctx& c, int nt) {
+  static void launch_face_bnd(Ctx& c, int nt) {
     FusedState& st = fs(c);
     Work<R>& w = st.work<R>();
-    const int nFF = c.z.nFaceFpts;
-    if (nFF == 0) return;
-    hipLaunchKernelGGL((k_face_visc<D, P, R>), dim3(nblk(nFF, nt)), dim3(nt), 0, c.stream, nFF, st.pL, st.pR, st.bc,
-                       w.Ufe, w.dUfe, st.norme, st.dAe, w.Fne, w.FnInv, c.ph, st.bst);
+    if (st.nBnd == 0) return;
+    hipLaunchKernelGGL((k_face_bnd<D, P, R>), dim3(nblk(st.nBnd, nt)), dim3(nt), 0, c.stream, st.nBnd, st.bndT, st.pL,
+                       st.bc, w.Ufe, w.gBnd, w.nrm, w.Bve, c.ph, st.bst);
     FCHK(hipGetLastError());
   }
   static void launch_rk(Ctx& c, int nt, int mode, double a, double b, bool write0, int stage = 0) {
@@ -1092,7 +1242,9 @@ struct Runner {
                        a, b, w.divFe, w.Fne, w.disFne, st.detJe, c.d_dt, c.eleDt,
                        c.prm.dt_type == 2 ? 1 : (c.prm.dt_type == 1 && st.nMesh > 1 ? 2 : 0), st.Ue, st.U0e, st.Uacc,
                        write0 ? st.divF0e : nullptr, w.Ufe, st.tb, st.eleMesh, st.meshDt, st.active,
-                       st.use_src ? st.srce : nullptr, rst, stage, bs);
+                       st.use_src ? st.srce : nullptr, rst, stage, bs, st.fpart, w.dA,
+                       c.prm.viscous ? w.Bve : nullptr, w.Us,
+                       (mode == 2 || mode == 3 || (mode == 4 && stage == c.z.nRK - 1)) ? 1 : 0);
     FCHK(hipGetLastError());
   }
   static void extrap(Ctx& c) {
@@ -1134,9 +1286,9 @@ struct Runner {
                        : nm("fused_volume_inviscid", "mixed_volume_inviscid"));
       launch_volume(c, blk(c, kVolume));
     }
-    if (visc) {
-      Launch L(c, nm("fused_face_viscous", "mixed_face_viscous"));
-      launch_face_visc(c, blk(c, kFaceVisc));
+    if (visc && fs(c).nBnd > 0) {
+      Launch L(c, nm("fused_face_boundary", "mixed_face_boundary"));
+      launch_face_bnd(c, blk(c, kFaceBnd));
     }
   }
   static void rk(Ctx& c, int mode, double a, double b, bool write0, int stage) {
@@ -1163,13 +1315,13 @@ struct Runner {
     FCHK(hipEventCreate(&e1));
     const int cand[4] = {64, 128, 256, 512};
     for (int k = kExtrap; k <= kRk; k++) {
-      if (k == kFaceVisc && !visc) continue;
+      if (k == kFaceBnd && (!visc || st.nBnd == 0)) continue;
       auto run = [&](int nt) {
         switch (k) {
           case kExtrap: launch_extrap(c, nt); break;
           case kFaceInv: launch_face_inv(c, nt); break;
           case kVolume: launch_volume(c, nt); break;
-          case kFaceVisc: launch_face_visc(c, nt); break;
+          case kFaceBnd: launch_face_bnd(c, nt); break;
           default: launch_rk(c, nt, 1, 0.5, 1. / 3., false);
         }
       };

@@ -158,7 +158,7 @@ void fused_begin(Ctx& c, bool mixed, bool src = false);
 void fused_end(Ctx& c);                // export U, divF_spts[0] and wave speeds
 void fused_rk_step(Ctx& c, bool refresh, bool mixed);
 void fused_residual(Ctx& c, bool mixed);  // divF_spts[0] and wave speeds of the current U
-// block sizes of the fused kernels (extrapolate, face_inviscid, volume, face_viscous, rk)
+// block sizes of the fused kernels (extrapolate, face_inviscid, volume, face_boundary, rk)
 void fused_autotune(Ctx& c, bool mixed, int* out);
 void fused_set_blocks(Ctx& c, bool mixed, const int* nt);
 void fused_get_blocks(Ctx& c, bool mixed, int* out);

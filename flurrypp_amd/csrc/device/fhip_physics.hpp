@@ -3,7 +3,9 @@
 // LDG terms.  Templated on the dimension D (N = D+2 fields) and on the real
 // type T, so the same code serves the FP64 path and the FP32 flux stage of
 // the mixed-precision path.  Formulas and term order follow
-// src/flux.cpp:37-210, src/face.cpp:163-615 and src/boundFace.cpp:65-352.
+// src/flux.cpp:37-210, src/face.cpp:163-615 and src/boundFace.cpp:65-352,
+// except that the point fluxes multiply by one reciprocal of rho where the
+// reference divides by rho (a rounding-level difference).
 #pragma once
 
 #include <hip/hip_runtime.h>
@@ -28,14 +30,14 @@ __device__ __forceinline__ T smax(T a, T b) { return (a < b) ? b : a; }
 template <int D, class T>
 __device__ __forceinline__ void inviscid_flux(const T* U, T F[D][D + 2], T gamma) {
   if constexpr (D == 2) {
-    T rho = U[0], u = U[1] / rho, v = U[2] / rho, E = U[3];
+    T rho = U[0], ri = T(1) / rho, u = U[1] * ri, v = U[2] * ri, E = U[3];
     T p = (gamma - T(1)) * (E - T(0.5) * rho * (u * u + v * v));
     F[0][0] = rho * u;         F[1][0] = rho * v;
     F[0][1] = rho * u * u + p; F[1][1] = rho * u * v;
     F[0][2] = rho * v * u;     F[1][2] = rho * v * v + p;
     F[0][3] = (E + p) * u;     F[1][3] = (E + p) * v;
   } else {
-    T rho = U[0], u = U[1] / rho, v = U[2] / rho, w = U[3] / rho, E = U[4];
+    T rho = U[0], ri = T(1) / rho, u = U[1] * ri, v = U[2] * ri, w = U[3] * ri, E = U[4];
     T p = (gamma - T(1)) * (E - T(0.5) * rho * (u * u + v * v + w * w));
     F[0][0] = rho * u;         F[1][0] = rho * v;         F[2][0] = rho * w;
     F[0][1] = rho * u * u + p; F[1][1] = rho * u * v;     F[2][1] = rho * u * w;
@@ -46,32 +48,67 @@ __device__ __forceinline__ void inviscid_flux(const T* U, T F[D][D + 2], T gamma
 }
 
 // viscousFlux, src/flux.cpp:79-210, including the reference's 3-D stress
-// expressions (tauxz, tauyz and the x-flux of rho*w as written there).
+// expressions (tauxz, tauyz and the x-flux of rho*w as written there), in
+// three steps so that a caller can feed the gradient one direction at a time:
+// visc_point (primitive state), visc_dir (one direction's conservative
+// gradient to du, dv, [dw,] de along it), visc_from_prims (the flux).
 template <int D, class T>
-__device__ __forceinline__ void viscous_flux(const T* U, const T g[D][D + 2], T Fv[D][D + 2], const Phys& ph) {
-  T rho = U[0], u = U[1] / rho, v = U[2] / rho;
-  T e = U[D + 1] / rho - T(0.5) * (u * u + v * v);
-  T w = 0;
+struct ViscPt {
+  T rho, ri, u, v, w, e, q2;
+};
+
+template <int D, class T>
+__device__ __forceinline__ ViscPt<D, T> visc_point(const T* U) {
+  ViscPt<D, T> p;
+  p.rho = U[0];
+  p.ri = T(1) / p.rho;
+  p.u = U[1] * p.ri;
+  p.v = U[2] * p.ri;
+  p.e = U[D + 1] * p.ri - T(0.5) * (p.u * p.u + p.v * p.v);
+  p.w = 0;
   if constexpr (D == 3) {
-    w = U[3] / rho;
-    e -= T(0.5) * (w * w);
+    p.w = U[3] * p.ri;
+    p.e -= T(0.5) * (p.w * p.w);
+    p.q2 = p.u * p.u + p.v * p.v + p.w * p.w;
+  } else {
+    p.q2 = p.u * p.u + p.v * p.v;
   }
-  T dRho_dx = g[0][0], dRhoU_dx = g[0][1], dRhoV_dx = g[0][2], dE_dx = g[0][D + 1];
-  T dRho_dy = g[1][0], dRhoU_dy = g[1][1], dRhoV_dy = g[1][2], dE_dy = g[1][D + 1];
+  return p;
+}
+
+// dp = {du, dv, [dw,] de} along one direction from that direction's gradient
+// of the conserved variables gx[N].
+template <int D, class T>
+__device__ __forceinline__ void visc_dir(const ViscPt<D, T>& p, const T* gx, T* dp) {
+  const T dRho = gx[0];
+  const T du = (gx[1] - dRho * p.u) * p.ri, dv = (gx[2] - dRho * p.v) * p.ri;
+  T dK;
+  if constexpr (D == 3) {
+    const T dw = (gx[3] - dRho * p.w) * p.ri;
+    dK = T(0.5) * p.q2 * dRho + p.rho * (p.u * du + p.v * dv + p.w * dw);
+    dp[2] = dw;
+  } else {
+    dK = T(0.5) * p.q2 * dRho + p.rho * (p.u * du + p.v * dv);
+  }
+  dp[0] = du;
+  dp[1] = dv;
+  dp[D] = (gx[D + 1] - dK - dRho * p.e) * p.ri;
+}
+
+template <int D, class T>
+__device__ __forceinline__ void visc_from_prims(const ViscPt<D, T>& p, const T dp[D][D + 1], T Fv[D][D + 2],
+                                                const Phys& ph) {
+  const T u = p.u, v = p.v, w = p.w;
   T mu = T(ph.mu_inf);
   if (!ph.fix_vis) {
-    T rt = (T(ph.gamma) - T(1)) * e / T(ph.rt_inf);
+    T rt = (T(ph.gamma) - T(1)) * p.e / T(ph.rt_inf);
     mu *= T(pow(double(rt), 1.5)) * (T(1) + T(ph.c_sth)) / (rt + T(ph.c_sth));
   }
-  T du_dx = (dRhoU_dx - dRho_dx * u) / rho, du_dy = (dRhoU_dy - dRho_dy * u) / rho;
-  T dv_dx = (dRhoV_dx - dRho_dx * v) / rho, dv_dy = (dRhoV_dy - dRho_dy * v) / rho;
-  T kappa = (mu / T(ph.prandtl)) * T(ph.gamma);
+  const T kappa = mu * T(ph.gamma / ph.prandtl);
+  const T du_dx = dp[0][0], dv_dx = dp[0][1], du_dy = dp[1][0], dv_dy = dp[1][1];
   if constexpr (D == 2) {
-    T dK_dx = T(0.5) * (u * u + v * v) * dRho_dx + rho * (u * du_dx + v * dv_dx);
-    T dK_dy = T(0.5) * (u * u + v * v) * dRho_dy + rho * (u * du_dy + v * dv_dy);
-    T de_dx = (dE_dx - dK_dx - dRho_dx * e) / rho;
-    T de_dy = (dE_dy - dK_dy - dRho_dy * e) / rho;
-    T diag = (du_dx + dv_dy + T(0)) / T(3);
+    const T de_dx = dp[0][2], de_dy = dp[1][2];
+    T diag = (du_dx + dv_dy + T(0)) * T(1. / 3.);
     T tauxx = T(2) * mu * (du_dx - diag);
     T tauyy = T(2) * mu * (dv_dy - diag);
     T tauxy = mu * (du_dy + dv_dx);
@@ -80,21 +117,9 @@ __device__ __forceinline__ void viscous_flux(const T* U, const T g[D][D + 2], T 
     Fv[1][0] = 0; Fv[1][1] = -tauxy; Fv[1][2] = -tauyy;
     Fv[1][3] = -(u * tauxy + v * tauyy + w * T(0) + kappa * de_dy);
   } else {
-    T dRho_dz = g[2][0], dRhoU_dz = g[2][1], dRhoV_dz = g[2][2];
-    T dRhoW_dx = g[0][3], dRhoW_dy = g[1][3], dRhoW_dz = g[2][3], dE_dz = g[2][4];
-    T du_dz = (dRhoU_dz - dRho_dz * u) / rho;
-    T dv_dz = (dRhoV_dz - dRho_dz * v) / rho;
-    T dw_dx = (dRhoW_dx - dRho_dx * w) / rho;
-    T dw_dy = (dRhoW_dy - dRho_dy * w) / rho;
-    T dw_dz = (dRhoW_dz - dRho_dz * w) / rho;
-    T q2 = u * u + v * v + w * w;
-    T dK_dx = T(0.5) * q2 * dRho_dx + rho * (u * du_dx + v * dv_dx + w * dw_dx);
-    T dK_dy = T(0.5) * q2 * dRho_dy + rho * (u * du_dy + v * dv_dy + w * dw_dy);
-    T dK_dz = T(0.5) * q2 * dRho_dz + rho * (u * du_dz + v * dv_dz + w * dw_dz);
-    T de_dx = (dE_dx - dK_dx - dRho_dx * e) / rho;
-    T de_dy = (dE_dy - dK_dy - dRho_dy * e) / rho;
-    T de_dz = (dE_dz - dK_dz - dRho_dz * e) / rho;
-    T diag = (du_dx + dv_dy + dw_dz) / T(3);
+    const T du_dz = dp[2][0], dv_dz = dp[2][1], dw_dz = dp[2][2];
+    const T de_dx = dp[0][3], de_dy = dp[1][3], de_dz = dp[2][3];
+    T diag = (du_dx + dv_dy + dw_dz) * T(1. / 3.);
     T tauxx = T(2) * mu * (du_dx - diag);
     T tauyy = T(2) * mu * (dv_dy - diag);
     T tauxy = mu * (du_dy + dv_dx);
@@ -110,6 +135,15 @@ __device__ __forceinline__ void viscous_flux(const T* U, const T g[D][D + 2], T 
   }
 }
 
+template <int D, class T>
+__device__ __forceinline__ void viscous_flux(const T* U, const T g[D][D + 2], T Fv[D][D + 2], const Phys& ph) {
+  const ViscPt<D, T> p = visc_point<D, T>(U);
+  T dp[D][D + 1];
+#pragma unroll
+  for (int x = 0; x < D; x++) visc_dir<D, T>(p, g[x], dp[x]);
+  visc_from_prims<D, T>(p, dp, Fv, ph);
+}
+
 // Rusanov common flux (src/face.cpp:314-374).  Returns the wave speed that
 // the reference stores into the left element.
 template <int D, class T>
@@ -118,14 +152,14 @@ __device__ __forceinline__ T rusanov(const T* UL, const T* UR, const T* n, T* Fn
   T FL[D][N], FR[D][N];
   inviscid_flux<D, T>(UL, FL, gamma);
   inviscid_flux<D, T>(UR, FR, gamma);
-  T rhoL = UL[0], rhoR = UR[0];
-  T uL = UL[1] / rhoL, uR = UR[1] / rhoR, vL = UL[2] / rhoL, vR = UR[2] / rhoR;
+  T rhoL = UL[0], rhoR = UR[0], riL = T(1) / rhoL, riR = T(1) / rhoR;
+  T uL = UL[1] * riL, uR = UR[1] * riR, vL = UL[2] * riL, vR = UR[2] * riR;
   T pL, pR;
   if constexpr (D == 2) {
     pL = (gamma - T(1)) * (UL[3] - T(0.5) * rhoL * (uL * uL + vL * vL));
     pR = (gamma - T(1)) * (UR[3] - T(0.5) * rhoR * (uR * uR + vR * vR));
   } else {
-    T wL = UL[3] / rhoL, wR = UR[3] / rhoR;
+    T wL = UL[3] * riL, wR = UR[3] * riR;
     pL = (gamma - T(1)) * (UL[4] - T(0.5) * rhoL * (uL * uL + vL * vL + wL * wL));
     pR = (gamma - T(1)) * (UR[4] - T(0.5) * rhoR * (uR * uR + vR * vR + wR * wR));
   }
@@ -134,15 +168,15 @@ __device__ __forceinline__ T rusanov(const T* UL, const T* UR, const T* n, T* Fn
   for (int i = 0; i < N; i++) FnL[i] = FnR[i] = 0;
 #pragma unroll
   for (int d = 0; d < D; d++) {
-    vnL += n[d] * UL[d + 1] / rhoL;
-    vnR += n[d] * UR[d + 1] / rhoR;
+    vnL += n[d] * UL[d + 1] * riL;
+    vnR += n[d] * UR[d + 1] * riR;
 #pragma unroll
     for (int i = 0; i < N; i++) {
       FnL[i] += n[d] * FL[d][i];
       FnR[i] += n[d] * FR[d][i];
     }
   }
-  T csqL = smax(gamma * pL / rhoL, T(0)), csqR = smax(gamma * pR / rhoR, T(0));
+  T csqL = smax(gamma * pL * riL, T(0)), csqR = smax(gamma * pR * riR, T(0));
   T eigL = fabs(vnL) + sqrt(csqL), eigR = fabs(vnR) + sqrt(csqR);
   T eig = smax(eigL, eigR);
 #pragma unroll
@@ -325,46 +359,92 @@ __device__ __forceinline__ T ldg_pen(const T* n, T pen) {
   else return (n[0] + n[1] + T(1.4142135623730951) * n[2] < T(0)) ? -pen : pen;
 }
 
+// One side of the LDG flux: that side's viscous flux projected on n,
+// nF[k] = sum_d n[d] F[d][k], and in 3-D d12[k] = F[1][k] - F[2][k], which the
+// reference's jump term uses in place of the z-flux difference
+// (src/face.cpp:277).
+template <int D, class T>
+__device__ __forceinline__ void ldg_project(const T F[D][D + 2], const T* n, T* nF, T* d12) {
+  constexpr int N = D + 2;
+#pragma unroll
+  for (int k = 0; k < N; k++) {
+    T a = n[0] * F[0][k] + n[1] * F[1][k];
+    if constexpr (D == 3) {
+      a += n[2] * F[2][k];
+      d12[k] = F[1][k] - F[2][k];
+    }
+    nF[k] = a;
+  }
+}
+
+template <int D, class T>
+__device__ __forceinline__ void ldg_side(const T* U, const T g[D][D + 2], const T* n, T* nF, T* d12,
+                                         const Phys& ph) {
+  T F[D][D + 2];
+  viscous_flux<D, T>(U, g, F, ph);
+  ldg_project<D, T>(F, n, nF, d12);
+}
+
 // LDG common viscous flux dotted with n, added into Fn
-// (face::calcViscousFlux, src/face.cpp:216-293).  bnd: 0 interior,
-// 1 Dirichlet-type boundary, 2 adiabatic wall.
+// (face::calcViscousFlux, src/face.cpp:216-293), one side at a time.  With
+// Fc[d] = (FL[d] + FR[d]) / 2 + pf n[d] s + tau n[d] (UL - UR) and
+// s = n . (FL - FR) (z term FL[1] - FR[2]), s = nFL - nFR + n[2] d12L and
+//   sum_d Fc[d] n[d] = [nFL / 2 + pf |n|^2 (nFL + n[2] d12L) + tau |n|^2 UL]
+//                    + [nFR / 2 - pf |n|^2 nFR - tau |n|^2 UR],
+// so each side adds its bracket and only N values carry over between sides.
+// Adds one side's bracket for that side's viscous flux F at the point.
+template <int D, class T>
+__device__ __forceinline__ void ldg_add_flux(bool left, const T* U, const T F[D][D + 2], const T* n, T* Fn,
+                                             const Phys& ph) {
+  constexpr int N = D + 2;
+  T nF[N], d12[N];
+  ldg_project<D, T>(F, n, nF, d12);
+  T nn = n[0] * n[0] + n[1] * n[1];
+  if constexpr (D == 3) nn += n[2] * n[2];
+  const T pn = ldg_pen<D, T>(n, T(ph.pen_fact)) * nn, tn = T(ph.tau) * nn;
+  // one flux evaluation for both sides; only the combination differs
+#pragma unroll
+  for (int k = 0; k < N; k++) {
+    T s = nF[k];
+    if constexpr (D == 3) s += n[2] * d12[k];
+    Fn[k] += left ? T(0.5) * nF[k] + pn * s + tn * U[k] : (T(0.5) - pn) * nF[k] - tn * U[k];
+  }
+}
+
+template <int D, class T>
+__device__ __forceinline__ void ldg_add_side(bool left, const T* U, const T g[D][D + 2], const T* n, T* Fn,
+                                             const Phys& ph) {
+  T F[D][D + 2];
+  viscous_flux<D, T>(U, g, F, ph);
+  ldg_add_flux<D, T>(left, U, F, n, Fn, ph);
+}
+
+// Boundary faces: the boundary side's flux (the right state's gradient on an
+// adiabatic wall, else the left one's) plus the penalty tau n (UL - UR).
+template <int D, class T>
+__device__ __forceinline__ void ldg_boundary(const T* UL, const T* UR, const T* nFB, const T* n, T* Fn,
+                                             const Phys& ph) {
+  constexpr int N = D + 2;
+  T nn = n[0] * n[0] + n[1] * n[1];
+  if constexpr (D == 3) nn += n[2] * n[2];
+  const T tau = T(ph.tau);
+#pragma unroll
+  for (int k = 0; k < N; k++) Fn[k] += nFB[k] + tau * (UL[k] - UR[k]) * nn;
+}
+
+// bnd: 0 interior, 1 Dirichlet-type boundary, 2 adiabatic wall.
 template <int D, class T>
 __device__ __forceinline__ void ldg_flux(int bnd, const T* UL, const T* UR, const T gL[D][D + 2],
                                          const T gR[D][D + 2], const T* n, T* Fn, const Phys& ph) {
   constexpr int N = D + 2;
-  T Fc[D][N];
-  T tau = T(ph.tau);
   if (bnd) {
-    T FB[D][N];
-    if (bnd == 2) viscous_flux<D, T>(UR, gR, FB, ph);
-    else viscous_flux<D, T>(UL, gL, FB, ph);
-#pragma unroll
-    for (int d = 0; d < D; d++)
-#pragma unroll
-      for (int k = 0; k < N; k++) Fc[d][k] = FB[d][k] + tau * n[d] * (UL[k] - UR[k]);
+    T nFB[N], d12[N];
+    ldg_side<D, T>(bnd == 2 ? UR : UL, bnd == 2 ? gR : gL, n, nFB, d12, ph);
+    ldg_boundary<D, T>(UL, UR, nFB, n, Fn, ph);
   } else {
-    T FL[D][N], FR[D][N];
-    viscous_flux<D, T>(UL, gL, FL, ph);
-    viscous_flux<D, T>(UR, gR, FR, ph);
-    T pf = ldg_pen<D, T>(n, T(ph.pen_fact));
-#pragma unroll
-    for (int k = 0; k < N; k++) {
-      T dF0 = FL[0][k] - FR[0][k], dF1 = FL[1][k] - FR[1][k];
-      T dU = UL[k] - UR[k];
-      T s;
-      if constexpr (D == 2) s = n[0] * (dF0) + n[1] * (dF1);
-      else {
-        T dF2 = FL[1][k] - FR[2][k];  // src/face.cpp:277
-        s = n[0] * (dF0) + n[1] * (dF1) + n[2] * (dF2);
-      }
-#pragma unroll
-      for (int d = 0; d < D; d++) Fc[d][k] = T(0.5) * (FL[d][k] + FR[d][k]) + pf * n[d] * s + tau * n[d] * (dU);
-    }
+    ldg_add_side<D, T>(true, UL, gL, n, Fn, ph);
+    ldg_add_side<D, T>(false, UR, gR, n, Fn, ph);
   }
-#pragma unroll
-  for (int d = 0; d < D; d++)
-#pragma unroll
-    for (int k = 0; k < N; k++) Fn[k] += Fc[d][k] * n[d];
 }
 
 }  // namespace fhip

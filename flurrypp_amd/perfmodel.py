@@ -27,14 +27,12 @@ def fused_work(D: int, P: int, E: int, nFF: int, nFFb: int, viscous: bool, nRK: 
     F = 2 * D * P1 ** (D - 1)
     nFFi = nFF - nFFb
     out = {}
-    # face_inviscid: read UL, UR (interior), normal; write inviscid Fn (or Fn*dA
-    # on both sides), wave speed; 3 int32 per face flux point
-    rd = nFF * (N + D + 3 * B4 / B8) + nFFi * N
+    # face_inviscid: read UL, UR (interior), normal, dA L (+R); write Fn*dA L
+    # (+R), wave speed (+ dUc L (+R) in viscous runs); 3 int32 per face flux point
+    rd = nFF * (N + D + 1 + 3 * B4 / B8) + nFFi * (N + 1)
+    wr = nFF * (N + 1) + nFFi * N
     if viscous:
-        wr = nFF * (N + N + 1) + nFFi * N          # FnInv, dUc L (+R), waveSp
-    else:
-        rd += nFF + nFFi                             # dA L (+R)
-        wr = nFF * (N + 1) + nFFi * N                # Fn L (+R), waveSp
+        wr += nFF * N + nFFi * N
     out["fused_face_inviscid"] = Work((rd + wr) * B8, 0.0)
     # volume
     rd = E * (N * S + D * D * S)
@@ -42,19 +40,23 @@ def fused_work(D: int, P: int, E: int, nFF: int, nFFb: int, viscous: bool, nRK: 
     fl = E * N * (D * S * 2 * P1 + F * D * 2 * P1)   # divergence + normal-flux extrapolation
     if viscous:
         rd += E * (S + N * F)                        # detJ, dUc
-        wr += E * D * N * F                          # gradient at the faces
-        fl += E * N * (D * S * 2 * P1 + D * S * 2 * 2 * D + D * F * 2 * P1)
+        rd += E * F * (D + B4 / B8)                  # face normal, side/partner code
+        wr += E * N * F + nFFb * D * N               # side of the LDG flux; boundary gradient
+        fl += E * N * (D * S * 2 * P1 + D * S * 2 * 2 * D + (D + 1) * F * 2 * P1)
         out["fused_volume_viscous"] = Work((rd + wr) * B8, fl)
     else:
         out["fused_volume_inviscid"] = Work((rd + wr) * B8, fl)
-    if viscous:
-        rd = nFF * (N + D * N + N + D + 1 + 3 * B4 / B8) + nFFi * (N + D * N + 1)
-        wr = nFF * N + nFFi * N
-        out["fused_face_viscous"] = Work((rd + wr) * B8, 0.0)
+    if viscous and nFFb:
+        # boundary LDG flux: U, normal, gradient in; the point's flux part out
+        rd = nFFb * (N + D + D * N + 3 * B4 / B8)
+        wr = nFFb * N
+        out["fused_face_boundary"] = Work((rd + wr) * B8, 0.0)
     # rk update, averaged over the stages of one step
     per_stage = []
     for st in range(nRK):
         rd = E * (N * S + 2 * N * F + S)             # divF, Fn, disFn, detJ
+        if viscous:
+            rd += E * F * (2 * N + 1 + B4 / B8)      # both sides' LDG parts, dA, partner code
         wr = E * (N * S + N * F)                     # U, U at the faces
         if time_type >= 5:
             rd += E * N * S                          # U0

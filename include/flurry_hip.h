@@ -189,7 +189,8 @@ enum { FHIP_OPT_GRAPHS = 1 };
 int fhip_set_option(fhip_ctx* ctx, int option, int value);
 
 /* Block sizes (threads per block, 64..512) of the fused kernels, in the order
- * extrapolate, face_inviscid, volume, face_viscous, rk_update, for path
+ * extrapolate, face_inviscid, volume, face_boundary (viscous LDG flux at
+ * boundary points), rk_update, for path
  * FHIP_PATH_FUSED or FHIP_PATH_MIXED (no counterpart in the reference, which
  * has no kernels).  fhip_autotune times every valid size of every kernel on
  * the context's current data, keeps the fastest and returns them in
