@@ -108,7 +108,8 @@ __host__ __device__ constexpr int spt_lds(int D, int P1, int s) {
 template <class R>
 struct Work {
   R *divFe = nullptr, *Ufe = nullptr, *Fne = nullptr, *disFne = nullptr, *dUce = nullptr;
-  // viscous runs: each flux point's side of the LDG flux [ele][N][F], and the
+  // viscous runs: each flux point's side of the LDG flux [ele][N][F] (disFne
+  // then holds that side's part of the flux jump, +-B dA - disFn), and the
   // gradient at the boundary flux points [D*N][nBnd]
   R *Bve = nullptr, *gBnd = nullptr;
   R* Rst = nullptr;  // mixed path: residual of each classical RK stage but the last
@@ -297,6 +298,14 @@ __global__ __launch_bounds__(kMaxBlock) void k_face_inv(int nFF, const int* __re
   }
 }
 
+// Viscous runs, FP32: the volume kernel also folds its own side's LDG part
+// into the flux jump (disFne := +-B dA - disFn), so the RK kernel reads one
+// array less.  FP64 keeps disFn and B apart: its volume kernel is short of
+// registers, and reading B back at the end of the element costs it more than
+// the RK kernel saves (measured).
+template <class R>
+constexpr bool kJumpInVolume = sizeof(R) == 4;
+
 // Viscous runs: what the volume kernel leaves at the flux points in place of
 // the gradient there (see the face loop of volume_body).
 template <class R>
@@ -305,6 +314,7 @@ struct VFace {
   const R* nfe;        // [E][D][F] the face's normal (its left element's)
   R* Bve;              // [E][N][F] this side's part of the LDG flux . n
   R* gBnd;             // [D*N][nBnd] gradient at the boundary flux points
+  const R* dA;         // [E][F] face area factor
   int nBnd;
 };
 
@@ -382,7 +392,7 @@ __device__ __forceinline__ void volume_body(int E, const double* __restrict__ Ue
         const R* gw = tw<R>(tb.grd_w, tb.grd_wf) + (d * S + s) * P1;
         const int* ci = tb.cu_idx + (d * S + s) * 2;
         const R* cw = tw<R>(tb.cu_w, tb.cu_wf) + (d * S + s) * 2;
-#pragma unroll 1
+#pragma unroll
         for (int k = 0; k < N; k++) {
           R a = 0;
 #pragma unroll
@@ -418,7 +428,11 @@ __device__ __forceinline__ void volume_body(int E, const double* __restrict__ Ue
       const long long ee = e0 + bb;
       const int* xi = tb.ext_idx + f * P1;
       const R* xw = tw<R>(tb.ext_w, tb.ext_wf) + f * P1;
+      // side code and normal first: their latency overlaps the sums below
       const int code = vf.fpart[ee * F + f], side = code & 3;
+      R nv[D];
+#pragma unroll
+      for (int x = 0; x < D; x++) nv[x] = vf.nfe[(ee * D + x) * F + f];
       R g[D][N];
 #pragma unroll
       for (int x = 0; x < D; x++)
@@ -436,7 +450,7 @@ __device__ __forceinline__ void volume_body(int E, const double* __restrict__ Ue
 #pragma unroll
           for (int k = 0; k < N; k++) vf.gBnd[(x * N + k) * (long long)vf.nBnd + slot] = g[x][k];
       } else {
-        R u[N], nv[D], bv[N];
+        R u[N], bv[N];
 #pragma unroll
         for (int k = 0; k < N; k++) {
           R a = 0;
@@ -445,8 +459,6 @@ __device__ __forceinline__ void volume_body(int E, const double* __restrict__ Ue
           u[k] = a;
           bv[k] = 0;
         }
-#pragma unroll
-        for (int x = 0; x < D; x++) nv[x] = vf.nfe[(ee * D + x) * F + f];
         ldg_add_side<D, R>(side == 0, u, g, nv, bv, ph);
 #pragma unroll
         for (int k = 0; k < N; k++) vf.Bve[(ee * N + k) * F + f] = bv[k];
@@ -507,7 +519,11 @@ __device__ __forceinline__ void volume_body(int E, const double* __restrict__ Ue
       divFe[(e * N + k) * S + s] = tot;
     }
   }
-  // faces: discontinuous normal flux and (viscous) gradient
+  // faces: discontinuous normal flux.  With kJumpInVolume, viscous runs store
+  // in its place this side's part of the correction's flux jump,
+  // +-B dA - disFn (B as written by this thread in the face loop above, - on
+  // the right side; boundary points get - disFn here and + B dA from the
+  // boundary kernel).
   for (int i = threadIdx.x; i < nb * F; i += blockDim.x) {
     const int bb = i / F, f = i - bb * F;
     const long long ee = e0 + bb;
@@ -516,11 +532,21 @@ __device__ __forceinline__ void volume_body(int E, const double* __restrict__ Ue
     // normal direction is non-zero (the other terms are exact zeros)
     const int dn = tb.efn_dir[f];
     const R* ew = tw<R>(tb.efn_w, tb.efn_wf) + f * P1;
+    int side = 2;
+    R dA = 0, bv[N];
+    if constexpr (VISC && kJumpInVolume<R>) {
+      side = vf.fpart[ee * F + f] & 3;
+      dA = vf.dA[ee * F + f];
+      // loaded for every point (all in flight at once); unused at boundary points
+#pragma unroll
+      for (int k = 0; k < N; k++) bv[k] = vf.Bve[(ee * N + k) * F + f];
+    }
 #pragma unroll
     for (int k = 0; k < N; k++) {
       R a = 0;
 #pragma unroll
       for (int m = 0; m < P1; m++) a += ew[m] * sF[((bb * D + dn) * N + k) * S + xi[m]];
+      if constexpr (VISC && kJumpInVolume<R>) a = (side == 2) ? -a : (side == 0 ? bv[k] : -bv[k]) * dA - a;
       disFne[(ee * N + k) * F + f] = a;
     }
   }
@@ -550,13 +576,15 @@ __global__ __launch_bounds__(kMaxBlock) __attribute__((amdgpu_waves_per_eu(4))) 
 // LDG flux at the boundary flux points (face::calcViscousFlux with the
 // boundary state of boundFace::applyViscousBCs, src/face.cpp:216-293,
 // src/boundFace.cpp:316-352): one thread per boundary point, from the
-// gradient the volume kernel left there; the result goes where the volume
-// kernel puts an interior point's side of the flux.
+// gradient the volume kernel left there.  The result goes where the volume
+// kernel puts an interior point's side of the flux, or (kJumpInVolume) times
+// dA into the point's flux jump, which the volume kernel set to -disFn.
 template <int D, int P, class R>
 __global__ __launch_bounds__(kMaxBlock) void k_face_bnd(int nB, const int* __restrict__ bndT,
                                                    const int* __restrict__ pLa, const int* __restrict__ bca,
                                                    const R* __restrict__ Ufe, const R* __restrict__ gBnd,
-                                                   const R* __restrict__ norme, R* __restrict__ Bve, Phys ph,
+                                                   const R* __restrict__ norme, const R* __restrict__ dAe,
+                                                   R* __restrict__ Bve, R* __restrict__ Cne, Phys ph,
                                                    const double* __restrict__ bst) {
   using Sh = Shape<D, P>;
   constexpr int F = Sh::F, N = Sh::N;
@@ -581,8 +609,14 @@ __global__ __launch_bounds__(kMaxBlock) void k_face_bnd(int nB, const int* __res
 #pragma unroll
   for (int k = 0; k < N; k++) bv[k] = 0;
   ldg_boundary<D, R>(UL, UR, nF, n, bv, ph);
+  if constexpr (kJumpInVolume<R>) {
+    const R dA = dAe[pl];
 #pragma unroll
-  for (int k = 0; k < N; k++) Bve[bL + k * F] = bv[k];
+    for (int k = 0; k < N; k++) Cne[bL + k * F] = bv[k] * dA + Cne[bL + k * F];
+  } else {
+#pragma unroll
+    for (int k = 0; k < N; k++) Bve[bL + k * F] = bv[k];
+  }
 }
 
 // ---------------------------------------------------------------------------
@@ -617,7 +651,28 @@ __global__ __launch_bounds__(kMaxBlock) void k_rk(int E, int mode, double a, dou
   R* const sD = sU + EB * N * S;                  // [EB*N*F] Fn - disFn at the block's flux points
   const int e0 = blockIdx.x * EB, nb = min(EB, E - e0);
   const int b = threadIdx.x / S, s = threadIdx.x - b * S;
-  if (Bve) {
+  if (Bve && kJumpInVolume<R>) {
+    // viscous: disFne holds this side's jump part +-B dA - disFn; add the
+    // inviscid part (Fne, already times dA) and the partner side's B times dA,
+    // negated on the right side
+    for (int i = threadIdx.x; i < nb * F; i += blockDim.x) {
+      const int bb = i / F, f = i - bb * F;
+      const long long ef = (long long)(e0 + bb) * F + f, ob = (long long)(e0 + bb) * N * F + f;
+      const int code = fpart[ef], side = code & 3;
+      if (side != 2) {
+        const R dA = dAe[ef];
+        const long long p = code >> 2, ep = p / F, pb = ep * N * F + (p - ep * F);
+#pragma unroll
+        for (int k = 0; k < N; k++) {
+          const R bp = Bve[pb + k * F];
+          sD[(bb * N + k) * F + f] = (Fne[ob + k * F] + disFne[ob + k * F]) + (side == 0 ? bp : -bp) * dA;
+        }
+      } else {
+#pragma unroll
+        for (int k = 0; k < N; k++) sD[(bb * N + k) * F + f] = Fne[ob + k * F] + disFne[ob + k * F];
+      }
+    }
+  } else if (Bve) {
     // viscous: common flux = inviscid part (Fne, already times dA) plus both
     // sides' LDG parts times dA, negated on the right side
     for (int i = threadIdx.x; i < nb * F; i += blockDim.x) {
@@ -1210,7 +1265,7 @@ struct Runner {
   static void launch_volume_t(Ctx& c, int nt) {
     FusedState& st = fs(c);
     Work<R>& w = st.work<R>();
-    const VFace<R> vf{st.fpart, w.nf, w.Bve, w.gBnd, st.nBnd};
+    const VFace<R> vf{st.fpart, w.nf, w.Bve, w.gBnd, w.dA, st.nBnd};
     // FP64 hexes above P3 need more than 128 VGPRs: no cap there
     if constexpr (D == 2 || P <= 3 || sizeof(R) == 4)
       hipLaunchKernelGGL((k_volume_w4<D, P, VISC, R>), dim3(nblk(c.z.E, nt / S)), dim3(nt), lds(kVolume, nt, VISC),
@@ -1229,7 +1284,7 @@ struct Runner {
     Work<R>& w = st.work<R>();
     if (st.nBnd == 0) return;
     hipLaunchKernelGGL((k_face_bnd<D, P, R>), dim3(nblk(st.nBnd, nt)), dim3(nt), 0, c.stream, st.nBnd, st.bndT, st.pL,
-                       st.bc, w.Ufe, w.gBnd, w.nrm, w.Bve, c.ph, st.bst);
+                       st.bc, w.Ufe, w.gBnd, w.nrm, w.dA, w.Bve, w.disFne, c.ph, st.bst);
     FCHK(hipGetLastError());
   }
   static void launch_rk(Ctx& c, int nt, int mode, double a, double b, bool write0, int stage = 0) {

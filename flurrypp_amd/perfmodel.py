@@ -1,4 +1,4 @@
-"""Algorithmic work of each libflurry_hip kernel, for roofline fractions.
+"""Algorithmic work of each libflurry_hip kernel (FP64 fused path), for roofline fractions.
 
 Bytes count every array a kernel must read or write once at its storage
 precision (FP64 state, metrics and fluxes; int32 face indices); flops count
