@@ -464,13 +464,16 @@ __device__ __forceinline__ void volume_body(int E, const double* __restrict__ Ue
         for (int k = 0; k < N; k++) vf.Bve[(ee * N + k) * F + f] = bv[k];
       }
     }
-    __syncthreads();
     // the metric terms again (from cache): keeping them live across the face
     // loop above costs more registers than the hold allows
     if (act) load_G();
   }
+  // phase 2: flux at the point, transformed to the reference element.  It is
+  // formed in registers and stored after a barrier: viscous runs store it over
+  // the gradient slab, which other threads may still be reading in the face
+  // loop, and threads with fewer face points go on here meanwhile.
+  R Ft[D][N];
   if (act) {
-    // phase 2: flux at the point, transformed to the reference element
     R u[N];
 #pragma unroll
     for (int k = 0; k < N; k++) u[k] = sU[(b * N + k) * S + sw];
@@ -498,8 +501,15 @@ __device__ __forceinline__ void volume_body(int E, const double* __restrict__ Ue
         R acc = 0;
 #pragma unroll
         for (int d2 = 0; d2 < D; d2++) acc += G[d1][d2] * Fl[d2][k];
-        sF[((b * D + d1) * N + k) * S + sw] = acc;
+        Ft[d1][k] = acc;
       }
+  }
+  if constexpr (VISC) __syncthreads();
+  if (act) {
+#pragma unroll
+    for (int d1 = 0; d1 < D; d1++)
+#pragma unroll
+      for (int k = 0; k < N; k++) sF[((b * D + d1) * N + k) * S + sw] = Ft[d1][k];
   }
   __syncthreads();
   if (act) {
