@@ -351,8 +351,11 @@ __device__ __forceinline__ void volume_body(int E, const double* __restrict__ Ue
 #pragma unroll
       for (int x = 0; x < D; x++) G[r][x] = JGe[((e * D + r) * D + x) * S + s];
   };
-  if constexpr (!VISC) {
-    if (act) load_G();
+  // metric loads in flight while the slab is staged and the gradient formed
+  R inv = 0;
+  if (act) {
+    load_G();
+    if constexpr (VISC) inv = iJe[e * S + s];
   }
   {
     if (Us) {
@@ -381,11 +384,7 @@ __device__ __forceinline__ void volume_body(int E, const double* __restrict__ Ue
     // src/solver.cpp:1105-1136), kept in LDS for phase 2 and the faces
     R* const sC = sF;
     R gr[D][N];
-    R inv = 0;
     if (act) {
-      // metric loads in flight while the reference gradient is formed
-      load_G();
-      inv = iJe[e * S + s];
 #pragma unroll
       for (int d = 0; d < D; d++) {
         const int* gi = tb.grd_idx + (d * S + s) * P1;
@@ -404,17 +403,25 @@ __device__ __forceinline__ void volume_body(int E, const double* __restrict__ Ue
           gr[d][k] = c + a;
         }
       }
+      // physical gradient, in place
+#pragma unroll
+      for (int k = 0; k < N; k++) {
+        R acc[D];
+#pragma unroll
+        for (int x = 0; x < D; x++) {
+          acc[x] = gr[0][k] * G[0][x] + gr[1][k] * G[1][x];
+          if constexpr (D == 3) acc[x] += gr[2][k] * G[2][x];
+        }
+#pragma unroll
+        for (int x = 0; x < D; x++) gr[x][k] = inv * acc[x];
+      }
     }
     __syncthreads();  // sC is overwritten by the gradient below
     if (act) {
 #pragma unroll
       for (int k = 0; k < N; k++)
 #pragma unroll
-        for (int x = 0; x < D; x++) {
-          R acc = gr[0][k] * G[0][x] + gr[1][k] * G[1][x];
-          if constexpr (D == 3) acc += gr[2][k] * G[2][x];
-          sG[((b * D + x) * N + k) * S + sw] = inv * acc;
-        }
+        for (int x = 0; x < D; x++) sG[((b * D + x) * N + k) * S + sw] = gr[x][k];
     }
     __syncthreads();
     // physical gradient at the flux points (extrapolateGradU) and, with U
