@@ -172,26 +172,43 @@ template <>
 Work<float>& FusedState::work<float>() { return w32; }
 
 // ---------------------------------------------------------------------------
-// Layout conversions between FlurryPP's [pt][ele][k] and element-major.
-__global__ void k_to_em(long long total, int P, int E, int K, const double* __restrict__ in, double* __restrict__ out) {
-  // in [P][E][K] -> out [E][K][P]
-  long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x;
-  if (i >= total) return;
-  int p = (int)(i % P);
-  long long r = i / P;
-  int k = (int)(r % K);
-  long long e = r / K;
-  out[i] = in[((long long)p * E + e) * K + k];
+// Layout conversions between FlurryPP's [pt][ele][k] and element-major
+// [ele][k][pt]: both are the transpose of a matrix with one row per point,
+// [P][E*K] <-> [E*K][P].  32x32 tiles through LDS so that reads and writes
+// are both coalesced; tiles are numbered on a 1-D grid.
+constexpr int kTile = 32;
+__global__ __launch_bounds__(256) void k_transpose(long long R, long long C, const double* __restrict__ in,
+                                                   double* __restrict__ out) {
+  __shared__ double t[kTile][kTile + 1];
+  const long long tilesC = (C + kTile - 1) / kTile;
+  const long long r0 = (blockIdx.x / tilesC) * kTile, c0 = (blockIdx.x % tilesC) * kTile;
+  const int tx = threadIdx.x % kTile, ty = threadIdx.x / kTile;  // ty 0..7
+#pragma unroll
+  for (int j = 0; j < kTile; j += 8) {
+    const long long r = r0 + ty + j, c = c0 + tx;
+    if (r < R && c < C) t[ty + j][tx] = in[r * C + c];
+  }
+  __syncthreads();
+#pragma unroll
+  for (int j = 0; j < kTile; j += 8) {
+    const long long c = c0 + ty + j, r = r0 + tx;
+    if (r < R && c < C) out[c * R + r] = t[tx][ty + j];
+  }
 }
-__global__ void k_from_em(long long total, int P, int E, int K, const double* __restrict__ in, double* __restrict__ out) {
-  // in [E][K][P] -> out [P][E][K]
-  long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x;
-  if (i >= total) return;
-  int k = (int)(i % K);
-  long long r = i / K;
-  long long e = r % E;
-  int p = (int)(r / E);
-  out[i] = in[(e * K + k) * P + p];
+
+// in [P][E][K] -> out [E][K][P]
+static void to_em(hipStream_t st, int P, long long E, int K, const double* in, double* out) {
+  const long long R = P, C = E * K;
+  const long long tiles = ((R + kTile - 1) / kTile) * ((C + kTile - 1) / kTile);
+  hipLaunchKernelGGL(k_transpose, dim3((unsigned)tiles), dim3(256), 0, st, R, C, in, out);
+  FCHK(hipGetLastError());
+}
+// in [E][K][P] -> out [P][E][K]
+static void from_em(hipStream_t st, int P, long long E, int K, const double* in, double* out) {
+  const long long R = E * K, C = P;
+  const long long tiles = ((R + kTile - 1) / kTile) * ((C + kTile - 1) / kTile);
+  hipLaunchKernelGGL(k_transpose, dim3((unsigned)tiles), dim3(256), 0, st, R, C, in, out);
+  FCHK(hipGetLastError());
 }
 
 // ---------------------------------------------------------------------------
@@ -1532,7 +1549,7 @@ void fused_begin(Ctx& c, bool mixed, bool src) {
   Ops o = get_ops(c, mixed);
   o.ensure(c);
   long long nS = (long long)z.E * z.N * z.S;
-  hipLaunchKernelGGL(k_to_em, dim3(nblk(nS, 256)), dim3(256), 0, c.stream, nS, z.S, z.E, z.N, c.U, st.Ue);
+  to_em(c.stream, z.S, z.E, z.N, c.U, st.Ue);
   st.use_src = src;
   if (src) {
     if (!c.src) throw std::invalid_argument("no P-multigrid source term on this context");
@@ -1541,10 +1558,9 @@ void fused_begin(Ctx& c, bool mixed, bool src) {
       FCHK(hipMalloc(&st.srce, nS * sizeof(double)));
       poison_fill(st.srce, nS * sizeof(double));
     }
-    hipLaunchKernelGGL(k_to_em, dim3(nblk(nS, 256)), dim3(256), 0, c.stream, nS, z.S, z.E, z.N, c.src, st.srce);
+    to_em(c.stream, z.S, z.E, z.N, c.src, st.srce);
   }
-  long long nW = (long long)z.E * z.F;
-  hipLaunchKernelGGL(k_to_em, dim3(nblk(nW, 256)), dim3(256), 0, c.stream, nW, z.F, z.E, 1, c.waveSp, st.waveSpe);
+  to_em(c.stream, z.F, z.E, 1, c.waveSp, st.waveSpe);
   FCHK(hipGetLastError());
   o.extrap(c);
 }
@@ -1553,11 +1569,9 @@ void fused_end(Ctx& c) {
   FusedState& st = fs(c);
   st.use_src = false;
   const Sizes& z = c.z;
-  long long nS = (long long)z.E * z.N * z.S;
-  hipLaunchKernelGGL(k_from_em, dim3(nblk(nS, 256)), dim3(256), 0, c.stream, nS, z.S, z.E, z.N, st.Ue, c.U);
-  hipLaunchKernelGGL(k_from_em, dim3(nblk(nS, 256)), dim3(256), 0, c.stream, nS, z.S, z.E, z.N, st.divF0e, c.divF);
-  long long nW = (long long)z.E * z.F;
-  hipLaunchKernelGGL(k_from_em, dim3(nblk(nW, 256)), dim3(256), 0, c.stream, nW, z.F, z.E, 1, st.waveSpe, c.waveSp);
+  from_em(c.stream, z.S, z.E, z.N, st.Ue, c.U);
+  from_em(c.stream, z.S, z.E, z.N, st.divF0e, c.divF);
+  from_em(c.stream, z.F, z.E, 1, st.waveSpe, c.waveSp);
   FCHK(hipGetLastError());
 }
 
@@ -1570,10 +1584,8 @@ void fused_residual(Ctx& c, bool mixed) {
   fused_begin(c, mixed, false);
   o.residual(c);
   o.rk(c, 5, 0., 0., true, 0);
-  long long nS = (long long)z.E * z.N * z.S;
-  hipLaunchKernelGGL(k_from_em, dim3(nblk(nS, 256)), dim3(256), 0, c.stream, nS, z.S, z.E, z.N, st.divF0e, c.divF);
-  long long nW = (long long)z.E * z.F;
-  hipLaunchKernelGGL(k_from_em, dim3(nblk(nW, 256)), dim3(256), 0, c.stream, nW, z.F, z.E, 1, st.waveSpe, c.waveSp);
+  from_em(c.stream, z.S, z.E, z.N, st.divF0e, c.divF);
+  from_em(c.stream, z.F, z.E, 1, st.waveSpe, c.waveSp);
   FCHK(hipGetLastError());
 }
 
