@@ -37,6 +37,7 @@ CASES = {
     "vortex2d_p5_jitter": lambda: cases.vortex2d(n=4, order=5),
     "bump2d_p5_char_slip_local_dt": lambda: cases.bump2d(nx=6, ny=4, order=5),
     "tgv3d_p5_ns_jitter": lambda: cases.tgv3d(n=2, order=5, viscous=1, jitter=0.1),
+    "box3d_p2_ns_walls": lambda: _box_ns(),
 }
 
 
@@ -61,6 +62,25 @@ def _bump_ns():
                                     (bc_number("adiabatic_noslip"), np.array(walls))])
     s = HostSetup(p, m2)
     return s, s.initial_condition()
+
+
+def _box_ns():
+    """3-D viscous box with every viscous boundary kind: isothermal no-slip
+    floor, moving isothermal lid, adiabatic no-slip and slip side walls,
+    characteristic ends; Sutherland viscosity, LDG penalty 0.5."""
+    from flurrypp_amd.input import InputParams
+    from flurrypp_amd.mesh import Mesh
+    from flurrypp_amd.solver import HostSetup
+    p = InputParams.from_dict(dict(equation=1, nDims=3, order=2, meshType=1, nx=3, ny=3, nz=3, xmin=0, xmax=1, ymin=0,
+                                   ymax=1, zmin=0, zmax=1, timeType=4, dtType=1, CFL=0.5, maxTime=1e9, iterMax=10,
+                                   icType=0, viscous=1, Re=100, MachBound=0.3, MachWall=0.1, fixVis=0,
+                                   LDG_penFact=0.5, LDG_tau=1.0, TWall=320.0))
+    m = Mesh.cartesian(3, (3, 3, 3), (0, 0, 0), (1, 1, 1),
+                       bcs={"bottom": "isothermal_noslip", "top": "isothermal_noslip_moving",
+                            "front": "adiabatic_noslip", "back": "slip_wall", "left": "char", "right": "char"})
+    s = HostSetup(p, m)
+    U = s.initial_condition()
+    return s, U * (1 + 0.01 * np.sin(np.arange(U.size)).reshape(U.shape))
 
 
 @pytest.fixture(scope="module", params=list(CASES))
@@ -131,7 +151,7 @@ MIXED_TOL = 1e-6  # relative L2 of the state against the FP64 oracle
 
 
 @pytest.mark.parametrize("name", ["vortex2d_p3_jitter_rusanov", "tgv3d_p3_ns_jitter", "bump2d_p3_ns_adiabatic",
-                                  "vortex2d_p2_roe"])
+                                  "vortex2d_p2_roe", "box3d_p2_ns_walls"])
 def test_mixed_precision_error(name):
     """Mixed path (FP32 fluxes/intermediates, FP64 state and RK sums) stays
     within 1e-6 relative L2 of the FP64 oracle after 10 RK steps."""
