@@ -113,6 +113,49 @@ def served_form(n_req: int, rate: float, seed: int) -> dict:
     return st
 
 
+def tgv_mixed_form(n: int, order: int, steps: int, warmup: int, tune: bool) -> dict:
+    """The Taylor-Green NS case at n^3 on the mixed path beside the FP64 path,
+    both from the same initial state: throughput of each and the mixed
+    path's error against the FP64 one (density L2, kinetic-energy
+    dissipation rate)."""
+    from flurrypp_amd import cases
+    from flurrypp_amd._native import PATH_FUSED, PATH_MIXED
+    from flurrypp_amd.solver import Solver
+    t = time.perf_counter()
+    setup, U = cases.tgv3d(n=n, order=order, viscous=1)
+    s = Solver(setup.params, setup=setup)
+    s.initializeSolution(U)
+    host = time.perf_counter() - t
+    dof, nRK = setup.nEles * setup.nSpts, setup.params.nRKSteps
+    runs = {}
+    for name, path in (("fp64", PATH_FUSED), ("mixed", PATH_MIXED)):
+        s.set_U(U)
+        if tune:
+            s.autotune(path)
+        s.update(warmup, path=path)
+        s.sync()
+        t0 = time.perf_counter()
+        s.update(steps, path=path, sync=False)
+        s.sync()
+        runs[name] = (time.perf_counter() - t0, s.get_U())
+        log(f"bench: TGV {n}^3 {name}: {dof * nRK * steps / runs[name][0] / 1e9:.3f} GDOF-stage/s")
+    s.close()
+    (el64, U64), (el32, U32) = runs["fp64"], runs["mixed"]
+    T = (warmup + steps) * setup.params.dt
+    ek0, ek64, ek32 = (kinetic_energy(setup, x) for x in (U, U64, U32))
+    eps64, eps32 = -(ek64 - ek0) / T, -(ek32 - ek0) / T
+    return {
+        "workload": f"3D Navier-Stokes Taylor-Green vortex, {n}^3 periodic hex, P{order}, Rusanov+LDG, RK44",
+        "elements": setup.nEles, "dof": dof, "host_setup_s": round(host, 2),
+        "value": round(dof * nRK * steps / el32 / 1e9, 4), "unit": "GDOF·stage/s",
+        "fp64_value": round(dof * nRK * steps / el64 / 1e9, 4), "speedup_vs_fp64": round(el64 / el32, 3),
+        "rel_l2_density_vs_fp64": float(np.linalg.norm(U32[..., 0] - U64[..., 0]) / np.linalg.norm(U64[..., 0])),
+        "rel_l2_state_vs_fp64": float(np.linalg.norm(U32 - U64) / np.linalg.norm(U64)),
+        "ke_dissipation_rel_err": abs(eps32 - eps64) / abs(eps64) if eps64 else None,
+        "after_steps": warmup + steps,
+    }
+
+
 def kinetic_energy(setup, U) -> float:
     """E_k = (1/|Omega|) sum w detJ rho |u|^2 / 2 over solution points (FP64)."""
     D = setup.nDims
@@ -149,6 +192,8 @@ def main() -> int:
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-mixed", action="store_true")
     ap.add_argument("--no-tune", action="store_true", help="keep the default block sizes (256)")
+    ap.add_argument("--mixed-n", type=int, default=64,
+                    help="elements per direction of the large mixed-precision Taylor-Green form (0 skips it)")
     ap.add_argument("--batch-cases", type=int, default=64, help="0 skips the batched form")
     ap.add_argument("--batch-tol", type=float, default=1e-10)
     ap.add_argument("--batch-max-steps", type=int, default=3000)
@@ -267,6 +312,9 @@ def main() -> int:
     s.copyUspts_U0()
     s.sync()
     s.close()
+    if a.path == "fused" and not a.no_mixed and a.mixed_n > 0:
+        log(f"bench: mixed-precision form at {a.mixed_n}^3")
+        forms[f"mixed_{a.mixed_n}"] = tgv_mixed_form(a.mixed_n, a.order, a.steps, a.warmup, not a.no_tune)
     if a.batch_cases > 0:
         log(f"bench: batched form ({a.batch_cases} cases)")
         forms["batched"] = batched_form(a.batch_cases, 11, a.batch_tol, a.batch_max_steps, a.batch_nx, a.batch_ny)
