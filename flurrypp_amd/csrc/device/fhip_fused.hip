@@ -290,7 +290,8 @@ __global__ __launch_bounds__(kMaxBlock) void k_face_inv(int nFF, const int* __re
 #pragma unroll
     for (int k = 0; k < N; k++) UR[k] = Ufe[bR + k * F];
   }
-  if (pr < 0 || ph.riemann_type == 0) waveSpe[pl] = double(rusanov<D, R>(UL, UR, n, fn, R(ph.gamma)));
+  if (pr < 0 || ph.riemann_type == 0)
+    __builtin_nontemporal_store(double(rusanov<D, R>(UL, UR, n, fn, R(ph.gamma))), &waveSpe[pl]);
   else {
     if constexpr (D == 2) roe2d<R>(UL, UR, n, fn, R(ph.gamma));
   }
@@ -298,19 +299,19 @@ __global__ __launch_bounds__(kMaxBlock) void k_face_inv(int nFF, const int* __re
   // LDG flux in the RK kernel, where both sides' parts meet
   const R dAL = dAe[pl];
 #pragma unroll
-  for (int k = 0; k < N; k++) Fne[bL + k * F] = fn[k] * dAL;
+  for (int k = 0; k < N; k++) __builtin_nontemporal_store(fn[k] * dAL, &Fne[bL + k * F]);
   if (pr >= 0) {
     const R dAR = dAe[pr];
 #pragma unroll
-    for (int k = 0; k < N; k++) Fne[bR + k * F] = -fn[k] * dAR;
+    for (int k = 0; k < N; k++) __builtin_nontemporal_store(-fn[k] * dAR, &Fne[bR + k * F]);
   }
   if (ph.viscous) {
     R pf = (pr < 0) ? R(0) : ldg_pen<D, R>(n, R(ph.pen_fact));
 #pragma unroll
     for (int k = 0; k < N; k++) {
       R uc = (pr < 0) ? R(0.5) * (UL[k] + UR[k]) : R(0.5) * (UL[k] + UR[k]) - pf * (UL[k] - UR[k]);
-      dUce[bL + k * F] = uc - UL[k];
-      if (pr >= 0) dUce[bR + k * F] = uc - UR[k];
+      __builtin_nontemporal_store(uc - UL[k], &dUce[bL + k * F]);
+      if (pr >= 0) __builtin_nontemporal_store(uc - UR[k], &dUce[bR + k * F]);
     }
   }
 }
@@ -379,19 +380,19 @@ __device__ __forceinline__ void volume_body(int E, const double* __restrict__ Ue
       const R* src = Us + (long long)e0 * N * S;
       for (int i = threadIdx.x; i < nb * N * S; i += blockDim.x) {
         const int r = i / S, q = i - r * S;
-        sU[r * S + spt_lds(D, P1, q)] = src[i];
+        sU[r * S + spt_lds(D, P1, q)] = __builtin_nontemporal_load(&src[i]);
       }
     } else {
       const double* src = Ue + (long long)e0 * N * S;
       for (int i = threadIdx.x; i < nb * N * S; i += blockDim.x) {
         const int r = i / S, q = i - r * S;
-        sU[r * S + spt_lds(D, P1, q)] = R(src[i]);
+        sU[r * S + spt_lds(D, P1, q)] = R(__builtin_nontemporal_load(&src[i]));
       }
     }
     if constexpr (VISC) {
       // dUc at the block's flux points (sF serves as sC until the gradient is written)
       const long long o = (long long)e0 * N * F;
-      for (int i = threadIdx.x; i < nb * N * F; i += blockDim.x) sF[i] = dUce[o + i];
+      for (int i = threadIdx.x; i < nb * N * F; i += blockDim.x) sF[i] = __builtin_nontemporal_load(&dUce[o + i]);
     }
   }
   __syncthreads();
@@ -550,7 +551,7 @@ __device__ __forceinline__ void volume_body(int E, const double* __restrict__ Ue
         for (int m = 0; m < P1; m++) a += gw[m] * sF[((b * D + d) * N + k) * S + gi[m]];
         tot = (d == 0) ? a : a + tot;
       }
-      divFe[(e * N + k) * S + s] = tot;
+      __builtin_nontemporal_store(tot, &divFe[(e * N + k) * S + s]);
     }
   }
   // faces: discontinuous normal flux.  With kJumpInVolume, viscous runs store
@@ -581,7 +582,7 @@ __device__ __forceinline__ void volume_body(int E, const double* __restrict__ Ue
 #pragma unroll
       for (int m = 0; m < P1; m++) a += ew[m] * sF[((bb * D + dn) * N + k) * S + xi[m]];
       if constexpr (VISC && kJumpInVolume<R>) a = (side == 2) ? -a : (side == 0 ? bv[k] : -bv[k]) * dA - a;
-      disFne[(ee * N + k) * F + f] = a;
+      __builtin_nontemporal_store(a, &disFne[(ee * N + k) * F + f]);
     }
   }
 }
@@ -665,6 +666,9 @@ __global__ __launch_bounds__(kMaxBlock) void k_face_bnd(int nB, const int* __res
 struct RkB {
   double b[8];
 };
+// Arrays that a kernel reads or writes exactly once per launch go through
+// nontemporal loads and stores (they stream past the L2); arrays read twice
+// (U at the flux points, the metric terms, the LDG side parts) stay cached.
 template <int D, int P, class R>
 __global__ __launch_bounds__(kMaxBlock) void k_rk(int E, int mode, double a, double bcoef, const R* __restrict__ divFe,
                                              const R* __restrict__ Fne, const R* __restrict__ disFne,
@@ -699,11 +703,13 @@ __global__ __launch_bounds__(kMaxBlock) void k_rk(int E, int mode, double a, dou
 #pragma unroll
         for (int k = 0; k < N; k++) {
           const R bp = Bve[pb + k * F];
-          sD[(bb * N + k) * F + f] = (Fne[ob + k * F] + disFne[ob + k * F]) + (side == 0 ? bp : -bp) * dA;
+          sD[(bb * N + k) * F + f] = (__builtin_nontemporal_load(&Fne[ob + k * F]) +
+                                      __builtin_nontemporal_load(&disFne[ob + k * F])) + (side == 0 ? bp : -bp) * dA;
         }
       } else {
 #pragma unroll
-        for (int k = 0; k < N; k++) sD[(bb * N + k) * F + f] = Fne[ob + k * F] + disFne[ob + k * F];
+        for (int k = 0; k < N; k++)
+          sD[(bb * N + k) * F + f] = __builtin_nontemporal_load(&Fne[ob + k * F]) + __builtin_nontemporal_load(&disFne[ob + k * F]);
       }
     }
   } else if (Bve) {
@@ -724,12 +730,14 @@ __global__ __launch_bounds__(kMaxBlock) void k_rk(int E, int mode, double a, dou
         R bv = Bve[ob + k * F];
         if (side != 2) bv += Bve[pb + k * F];
         if (side == 1) bv = -bv;
-        sD[(bb * N + k) * F + f] = (Fne[ob + k * F] + bv * dA) - disFne[ob + k * F];
+        sD[(bb * N + k) * F + f] = (__builtin_nontemporal_load(&Fne[ob + k * F]) + bv * dA) -
+                                   __builtin_nontemporal_load(&disFne[ob + k * F]);
       }
     }
   } else {
     const long long o = (long long)e0 * N * F;
-    for (int i = threadIdx.x; i < nb * N * F; i += blockDim.x) sD[i] = Fne[o + i] - disFne[o + i];
+    for (int i = threadIdx.x; i < nb * N * F; i += blockDim.x)
+      sD[i] = __builtin_nontemporal_load(&Fne[o + i]) - __builtin_nontemporal_load(&disFne[o + i]);
   }
   __syncthreads();
   if (mode == 5) {
@@ -741,7 +749,7 @@ __global__ __launch_bounds__(kMaxBlock) void k_rk(int E, int mode, double a, dou
         R c = 0;
 #pragma unroll
         for (int j = 0; j < NC; j++) c += tw<R>(tb.cor_w, tb.cor_wf)[s * NC + j] * sD[(b * N + k) * F + tb.cor_idx[s * NC + j]];
-        divF0e[(e * N + k) * S + s] = double(c + divFe[(e * N + k) * S + s]);
+        __builtin_nontemporal_store(double(c + __builtin_nontemporal_load(&divFe[(e * N + k) * S + s])), &divF0e[(e * N + k) * S + s]);
       }
     }
     return;  // uniform over the block: no barrier follows for anyone
@@ -751,49 +759,49 @@ __global__ __launch_bounds__(kMaxBlock) void k_rk(int E, int mode, double a, dou
     // dt: 0 global, 1 per element (local CFL), 2 per mesh (batched global CFL)
     const double dt = (local == 1) ? eleDt[e] : (local == 2 ? meshDt[eleMesh[e]] : *dtp);
     const bool frozen = active && !active[eleMesh[e]];
-    const double dj = detJe[e * S + s];
+    const double dj = __builtin_nontemporal_load(&detJe[e * S + s]);
 #pragma unroll
     for (int k = 0; k < N; k++) {
       const long long iu = (e * N + k) * S + s;
       R c = 0;
 #pragma unroll
       for (int j = 0; j < NC; j++) c += tw<R>(tb.cor_w, tb.cor_wf)[s * NC + j] * sD[(b * N + k) * F + tb.cor_idx[s * NC + j]];
-      const double r0 = double(c + divFe[iu]);
-      if (divF0e) divF0e[iu] = r0;
+      const double r0 = double(c + __builtin_nontemporal_load(&divFe[iu]));
+      if (divF0e) __builtin_nontemporal_store(r0, &divF0e[iu]);
       // P-multigrid source: divF_spts[step] + src_spts (src/solver.cpp:480,514)
       const double r = srce ? r0 + srce[iu] : r0;
       // U itself is an input of the first and the single stage only (and of
       // frozen batch members, which keep it)
-      const double u = (mode == 0 || mode == 3 || frozen) ? Ue[iu] : 0.0;
+      const double u = (mode == 0 || mode == 3 || frozen) ? __builtin_nontemporal_load(&Ue[iu]) : 0.0;
       double un;
       if (mode == 0) {
-        U0e[iu] = u;
-        if (Rst) Rst[iu] = R(r);
-        else Uacc[iu] = u - bcoef * r / dj * dt;
+        __builtin_nontemporal_store(u, &U0e[iu]);
+        if (Rst) __builtin_nontemporal_store(R(r), &Rst[iu]);
+        else __builtin_nontemporal_store(u - bcoef * r / dj * dt, &Uacc[iu]);
         un = u - a * r / dj * dt;
       } else if (mode == 1) {
-        if (Rst) Rst[stage * nS + iu] = R(r);
-        else Uacc[iu] = Uacc[iu] - bcoef * r / dj * dt;
-        un = U0e[iu] - a * r / dj * dt;
+        if (Rst) __builtin_nontemporal_store(R(r), &Rst[stage * nS + iu]);
+        else __builtin_nontemporal_store(__builtin_nontemporal_load(&Uacc[iu]) - bcoef * r / dj * dt, &Uacc[iu]);
+        un = __builtin_nontemporal_load(&U0e[iu]) - a * r / dj * dt;
       } else if (mode == 2) {
         double acc;
         if (Rst) {
-          acc = U0e[iu];
-          for (int j = 0; j < stage; j++) acc = acc - bs.b[j] * double(Rst[j * nS + iu]) / dj * dt;
+          acc = __builtin_nontemporal_load(&U0e[iu]);
+          for (int j = 0; j < stage; j++) acc = acc - bs.b[j] * double(__builtin_nontemporal_load(&Rst[j * nS + iu])) / dj * dt;
         } else {
-          acc = Uacc[iu];
+          acc = __builtin_nontemporal_load(&Uacc[iu]);
         }
         un = acc - bcoef * r / dj * dt;
       } else if (mode == 3) {
         un = u - bcoef * r / dj * dt;
       } else {
-        un = U0e[iu] - a * r / dj * dt;
+        un = __builtin_nontemporal_load(&U0e[iu]) - a * r / dj * dt;
       }
       if (frozen) un = u;
       // with an FP32 stage copy, the FP64 state is written at the step's
       // last stage only (earlier stages leave it at the step's start)
-      if (!Us || fin) Ue[iu] = un;
-      if (Us) Us[iu] = R(un);
+      if (!Us || fin) __builtin_nontemporal_store(un, &Ue[iu]);
+      if (Us) __builtin_nontemporal_store(R(un), &Us[iu]);
       sU[(b * N + k) * S + spt_lds(D, P1, s)] = R(un);
     }
   }
@@ -805,7 +813,7 @@ __global__ __launch_bounds__(kMaxBlock) void k_rk(int E, int mode, double a, dou
       R acc = 0;
 #pragma unroll
       for (int m = 0; m < P1; m++) acc += tw<R>(tb.ext_w, tb.ext_wf)[f * P1 + m] * sU[(bb * N + k) * S + tb.ext_idx[f * P1 + m]];
-      Ufe[((long long)(e0 + bb) * N + k) * F + f] = acc;
+      __builtin_nontemporal_store(acc, &Ufe[((long long)(e0 + bb) * N + k) * F + f]);
     }
   }
 }
