@@ -241,6 +241,15 @@ __global__ __launch_bounds__(kMaxBlock) void k_extrap(int E, const double* __res
   }
 }
 
+// U at a face flux point, read once per face kernel launch: nontemporal in
+// FP64; FP32 loads stay cached, since a 128-byte line then spans more face
+// points that other waves read (measured slower when streamed).
+template <class R>
+__device__ __forceinline__ R ld_face(const R* p) {
+  if constexpr (sizeof(R) == 8) return __builtin_nontemporal_load(p);
+  else return *p;
+}
+
 // ---------------------------------------------------------------------------
 // Common inviscid flux at face flux points; pL/pR are ele*F+fpt.
 // Boundary state of a face flux point: the packed code holds the BC number in
@@ -273,14 +282,15 @@ __global__ __launch_bounds__(kMaxBlock) void k_face_inv(int nFF, const int* __re
   constexpr int F = Sh::F, N = Sh::N;
   int t = blockIdx.x * blockDim.x + threadIdx.x;
   if (t >= nFF) return;
-  const int pl = pLa[t], pr = pRa[t], bc = bca[t];
+  const int pl = __builtin_nontemporal_load(&pLa[t]), pr = __builtin_nontemporal_load(&pRa[t]),
+            bc = __builtin_nontemporal_load(&bca[t]);
   const long long eL = pl / F, fL = pl - eL * F;
   const long long bL = eL * N * F + fL;
   R UL[N], UR[N], n[D], fn[N];
 #pragma unroll
-  for (int k = 0; k < N; k++) UL[k] = Ufe[bL + k * F];
+  for (int k = 0; k < N; k++) UL[k] = ld_face(&Ufe[bL + k * F]);
 #pragma unroll
-  for (int d = 0; d < D; d++) n[d] = norme[(eL * D + d) * F + fL];
+  for (int d = 0; d < D; d++) n[d] = __builtin_nontemporal_load(&norme[(eL * D + d) * F + fL]);
   long long bR = 0;
   if (pr < 0) {
     bc_state<D, R>(bc, UL, n, UR, ph, bst);
@@ -288,7 +298,7 @@ __global__ __launch_bounds__(kMaxBlock) void k_face_inv(int nFF, const int* __re
     const long long eR = pr / F, fR = pr - eR * F;
     bR = eR * N * F + fR;
 #pragma unroll
-    for (int k = 0; k < N; k++) UR[k] = Ufe[bR + k * F];
+    for (int k = 0; k < N; k++) UR[k] = ld_face(&Ufe[bR + k * F]);
   }
   if (pr < 0 || ph.riemann_type == 0)
     __builtin_nontemporal_store(double(rusanov<D, R>(UL, UR, n, fn, R(ph.gamma))), &waveSpe[pl]);
@@ -297,11 +307,11 @@ __global__ __launch_bounds__(kMaxBlock) void k_face_inv(int nFF, const int* __re
   }
   // the inviscid common flux times dA on both sides; viscous runs add the
   // LDG flux in the RK kernel, where both sides' parts meet
-  const R dAL = dAe[pl];
+  const R dAL = __builtin_nontemporal_load(&dAe[pl]);
 #pragma unroll
   for (int k = 0; k < N; k++) __builtin_nontemporal_store(fn[k] * dAL, &Fne[bL + k * F]);
   if (pr >= 0) {
-    const R dAR = dAe[pr];
+    const R dAR = __builtin_nontemporal_load(&dAe[pr]);
 #pragma unroll
     for (int k = 0; k < N; k++) __builtin_nontemporal_store(-fn[k] * dAR, &Fne[bR + k * F]);
   }
@@ -457,7 +467,7 @@ __device__ __forceinline__ void volume_body(int E, const double* __restrict__ Ue
       const int code = vf.fpart[ee * F + f], side = code & 3;
       R nv[D];
 #pragma unroll
-      for (int x = 0; x < D; x++) nv[x] = vf.nfe[(ee * D + x) * F + f];
+      for (int x = 0; x < D; x++) nv[x] = __builtin_nontemporal_load(&vf.nfe[(ee * D + x) * F + f]);
       R g[D][N];
 #pragma unroll
       for (int x = 0; x < D; x++)
@@ -486,7 +496,7 @@ __device__ __forceinline__ void volume_body(int E, const double* __restrict__ Ue
         }
         ldg_add_side<D, R>(side == 0, u, g, nv, bv, ph);
 #pragma unroll
-        for (int k = 0; k < N; k++) vf.Bve[(ee * N + k) * F + f] = bv[k];
+        for (int k = 0; k < N; k++) __builtin_nontemporal_store(bv[k], &vf.Bve[(ee * N + k) * F + f]);
       }
     }
     // the metric terms again (from cache): keeping them live across the face
@@ -668,7 +678,7 @@ struct RkB {
 };
 // Arrays that a kernel reads or writes exactly once per launch go through
 // nontemporal loads and stores (they stream past the L2); arrays read twice
-// (U at the flux points, the metric terms, the LDG side parts) stay cached.
+// (the metric terms, the LDG side parts) stay cached.
 template <int D, int P, class R>
 __global__ __launch_bounds__(kMaxBlock) void k_rk(int E, int mode, double a, double bcoef, const R* __restrict__ divFe,
                                              const R* __restrict__ Fne, const R* __restrict__ disFne,
