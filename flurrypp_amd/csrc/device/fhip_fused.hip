@@ -63,6 +63,7 @@ struct Shape {
 extern __shared__ __align__(16) unsigned char fhip_lds[];
 constexpr int kMaxBlock = 512;
 constexpr size_t kMaxLds = 64 * 1024;
+constexpr long long kStreamBytes = 64ll << 20;
 enum { kExtrap = 0, kFaceInv = 1, kVolume = 2, kFaceBnd = 3, kRk = 4 };
 
 // Sparse operator tables (device pointers into one allocation).  Indices of
@@ -151,6 +152,10 @@ struct FusedState {
   // P-multigrid source term (element-major copy of src_spts), used while use_src
   double* srce = nullptr;
   bool use_src = false;
+  // stream once-touched arrays past the caches: set when U alone (FP64,
+  // element-major) exceeds kStreamBytes, i.e. a stage's arrays are many times
+  // the 256 MiB infinity cache
+  bool stream = false;
   // block size of each kernel (kExtrap..kRk) for FP64 [0] and mixed [1]
   int nt[2][5] = {{256, 256, 256, 256, 256}, {256, 256, 256, 256, 256}};
   Work<double> w64;
@@ -241,13 +246,26 @@ __global__ __launch_bounds__(kMaxBlock) void k_extrap(int E, const double* __res
   }
 }
 
-// U at a face flux point, read once per face kernel launch: nontemporal in
-// FP64; FP32 loads stay cached, since a 128-byte line then spans more face
-// points that other waves read (measured slower when streamed).
-template <class R>
-__device__ __forceinline__ R ld_face(const R* p) {
-  if constexpr (sizeof(R) == 8) return __builtin_nontemporal_load(p);
+// Loads and stores of arrays a kernel touches once per launch.  On meshes
+// whose arrays are far larger than the caches (NT) they stream past them
+// (nontemporal); on smaller ones they stay plain, since there the next kernel
+// of the stage finds them in L2 or the infinity cache (measured both ways).
+template <bool NT, class T>
+__device__ __forceinline__ T ld_s(const T* p) {
+  if constexpr (NT) return __builtin_nontemporal_load(p);
   else return *p;
+}
+template <bool NT, class T>
+__device__ __forceinline__ void st_s(T v, T* p) {
+  if constexpr (NT) __builtin_nontemporal_store(v, p);
+  else *p = v;
+}
+// U at a face flux point, read once per face kernel launch: streamed in FP64
+// only; FP32 loads stay cached, since a 128-byte line then spans more face
+// points that other waves read (measured slower when streamed).
+template <bool NT, class R>
+__device__ __forceinline__ R ld_face(const R* p) {
+  return ld_s<NT && sizeof(R) == 8>(p);
 }
 
 // ---------------------------------------------------------------------------
@@ -271,7 +289,7 @@ __device__ __forceinline__ void bc_state(int code, const R* UL, const R* n, R* U
   }
 }
 
-template <int D, int P, class R>
+template <int D, int P, class R, bool NT>
 __global__ __launch_bounds__(kMaxBlock) void k_face_inv(int nFF, const int* __restrict__ pLa, const int* __restrict__ pRa,
                                                    const int* __restrict__ bca, const R* __restrict__ Ufe,
                                                    const R* __restrict__ norme, const R* __restrict__ dAe,
@@ -282,15 +300,15 @@ __global__ __launch_bounds__(kMaxBlock) void k_face_inv(int nFF, const int* __re
   constexpr int F = Sh::F, N = Sh::N;
   int t = blockIdx.x * blockDim.x + threadIdx.x;
   if (t >= nFF) return;
-  const int pl = __builtin_nontemporal_load(&pLa[t]), pr = __builtin_nontemporal_load(&pRa[t]),
-            bc = __builtin_nontemporal_load(&bca[t]);
+  const int pl = ld_s<NT>(&pLa[t]), pr = ld_s<NT>(&pRa[t]),
+            bc = ld_s<NT>(&bca[t]);
   const long long eL = pl / F, fL = pl - eL * F;
   const long long bL = eL * N * F + fL;
   R UL[N], UR[N], n[D], fn[N];
 #pragma unroll
-  for (int k = 0; k < N; k++) UL[k] = ld_face(&Ufe[bL + k * F]);
+  for (int k = 0; k < N; k++) UL[k] = ld_face<NT>(&Ufe[bL + k * F]);
 #pragma unroll
-  for (int d = 0; d < D; d++) n[d] = __builtin_nontemporal_load(&norme[(eL * D + d) * F + fL]);
+  for (int d = 0; d < D; d++) n[d] = ld_s<NT>(&norme[(eL * D + d) * F + fL]);
   long long bR = 0;
   if (pr < 0) {
     bc_state<D, R>(bc, UL, n, UR, ph, bst);
@@ -298,30 +316,30 @@ __global__ __launch_bounds__(kMaxBlock) void k_face_inv(int nFF, const int* __re
     const long long eR = pr / F, fR = pr - eR * F;
     bR = eR * N * F + fR;
 #pragma unroll
-    for (int k = 0; k < N; k++) UR[k] = ld_face(&Ufe[bR + k * F]);
+    for (int k = 0; k < N; k++) UR[k] = ld_face<NT>(&Ufe[bR + k * F]);
   }
   if (pr < 0 || ph.riemann_type == 0)
-    __builtin_nontemporal_store(double(rusanov<D, R>(UL, UR, n, fn, R(ph.gamma))), &waveSpe[pl]);
+    st_s<NT>(double(rusanov<D, R>(UL, UR, n, fn, R(ph.gamma))), &waveSpe[pl]);
   else {
     if constexpr (D == 2) roe2d<R>(UL, UR, n, fn, R(ph.gamma));
   }
   // the inviscid common flux times dA on both sides; viscous runs add the
   // LDG flux in the RK kernel, where both sides' parts meet
-  const R dAL = __builtin_nontemporal_load(&dAe[pl]);
+  const R dAL = ld_s<NT>(&dAe[pl]);
 #pragma unroll
-  for (int k = 0; k < N; k++) __builtin_nontemporal_store(fn[k] * dAL, &Fne[bL + k * F]);
+  for (int k = 0; k < N; k++) st_s<NT>(fn[k] * dAL, &Fne[bL + k * F]);
   if (pr >= 0) {
-    const R dAR = __builtin_nontemporal_load(&dAe[pr]);
+    const R dAR = ld_s<NT>(&dAe[pr]);
 #pragma unroll
-    for (int k = 0; k < N; k++) __builtin_nontemporal_store(-fn[k] * dAR, &Fne[bR + k * F]);
+    for (int k = 0; k < N; k++) st_s<NT>(-fn[k] * dAR, &Fne[bR + k * F]);
   }
   if (ph.viscous) {
     R pf = (pr < 0) ? R(0) : ldg_pen<D, R>(n, R(ph.pen_fact));
 #pragma unroll
     for (int k = 0; k < N; k++) {
       R uc = (pr < 0) ? R(0.5) * (UL[k] + UR[k]) : R(0.5) * (UL[k] + UR[k]) - pf * (UL[k] - UR[k]);
-      __builtin_nontemporal_store(uc - UL[k], &dUce[bL + k * F]);
-      if (pr >= 0) __builtin_nontemporal_store(uc - UR[k], &dUce[bR + k * F]);
+      st_s<NT>(uc - UL[k], &dUce[bL + k * F]);
+      if (pr >= 0) st_s<NT>(uc - UR[k], &dUce[bR + k * F]);
     }
   }
 }
@@ -350,7 +368,7 @@ struct VFace {
 // Volume kernel: [corrected gradient], flux, divergence, normal flux at faces
 // [and gradient at faces].  One thread per solution point, EB elements per
 // block, element data staged in LDS.
-template <int D, int P, bool VISC, class R>
+template <int D, int P, bool VISC, class R, bool NT>
 __device__ __forceinline__ void volume_body(int E, const double* __restrict__ Ue, const R* __restrict__ Us,
                                             const R* __restrict__ dUce,
                                             const R* __restrict__ JGe, const R* __restrict__ iJe,
@@ -390,19 +408,19 @@ __device__ __forceinline__ void volume_body(int E, const double* __restrict__ Ue
       const R* src = Us + (long long)e0 * N * S;
       for (int i = threadIdx.x; i < nb * N * S; i += blockDim.x) {
         const int r = i / S, q = i - r * S;
-        sU[r * S + spt_lds(D, P1, q)] = __builtin_nontemporal_load(&src[i]);
+        sU[r * S + spt_lds(D, P1, q)] = ld_s<NT>(&src[i]);
       }
     } else {
       const double* src = Ue + (long long)e0 * N * S;
       for (int i = threadIdx.x; i < nb * N * S; i += blockDim.x) {
         const int r = i / S, q = i - r * S;
-        sU[r * S + spt_lds(D, P1, q)] = R(__builtin_nontemporal_load(&src[i]));
+        sU[r * S + spt_lds(D, P1, q)] = R(ld_s<NT>(&src[i]));
       }
     }
     if constexpr (VISC) {
       // dUc at the block's flux points (sF serves as sC until the gradient is written)
       const long long o = (long long)e0 * N * F;
-      for (int i = threadIdx.x; i < nb * N * F; i += blockDim.x) sF[i] = __builtin_nontemporal_load(&dUce[o + i]);
+      for (int i = threadIdx.x; i < nb * N * F; i += blockDim.x) sF[i] = ld_s<NT>(&dUce[o + i]);
     }
   }
   __syncthreads();
@@ -467,7 +485,7 @@ __device__ __forceinline__ void volume_body(int E, const double* __restrict__ Ue
       const int code = vf.fpart[ee * F + f], side = code & 3;
       R nv[D];
 #pragma unroll
-      for (int x = 0; x < D; x++) nv[x] = __builtin_nontemporal_load(&vf.nfe[(ee * D + x) * F + f]);
+      for (int x = 0; x < D; x++) nv[x] = ld_s<NT>(&vf.nfe[(ee * D + x) * F + f]);
       R g[D][N];
 #pragma unroll
       for (int x = 0; x < D; x++)
@@ -496,7 +514,7 @@ __device__ __forceinline__ void volume_body(int E, const double* __restrict__ Ue
         }
         ldg_add_side<D, R>(side == 0, u, g, nv, bv, ph);
 #pragma unroll
-        for (int k = 0; k < N; k++) __builtin_nontemporal_store(bv[k], &vf.Bve[(ee * N + k) * F + f]);
+        for (int k = 0; k < N; k++) st_s<NT>(bv[k], &vf.Bve[(ee * N + k) * F + f]);
       }
     }
     // the metric terms again (from cache): keeping them live across the face
@@ -561,7 +579,7 @@ __device__ __forceinline__ void volume_body(int E, const double* __restrict__ Ue
         for (int m = 0; m < P1; m++) a += gw[m] * sF[((b * D + d) * N + k) * S + gi[m]];
         tot = (d == 0) ? a : a + tot;
       }
-      __builtin_nontemporal_store(tot, &divFe[(e * N + k) * S + s]);
+      st_s<NT>(tot, &divFe[(e * N + k) * S + s]);
     }
   }
   // faces: discontinuous normal flux.  With kJumpInVolume, viscous runs store
@@ -592,7 +610,7 @@ __device__ __forceinline__ void volume_body(int E, const double* __restrict__ Ue
 #pragma unroll
       for (int m = 0; m < P1; m++) a += ew[m] * sF[((bb * D + dn) * N + k) * S + xi[m]];
       if constexpr (VISC && kJumpInVolume<R>) a = (side == 2) ? -a : (side == 0 ? bv[k] : -bv[k]) * dA - a;
-      __builtin_nontemporal_store(a, &disFne[(ee * N + k) * F + f]);
+      st_s<NT>(a, &disFne[(ee * N + k) * F + f]);
     }
   }
 }
@@ -601,20 +619,20 @@ __device__ __forceinline__ void volume_body(int E, const double* __restrict__ Ue
 // 128 VGPRs) for the shapes that fit there without spilling: its LDS slabs
 // limit it to 4 waves per SIMD anyway, and the cap lets the metric terms stay
 // in registers across the whole element.
-template <int D, int P, bool VISC, class R>
+template <int D, int P, bool VISC, class R, bool NT>
 __global__ __launch_bounds__(kMaxBlock) void k_volume(int E, const double* __restrict__ Ue, const R* __restrict__ Us,
                                                  const R* __restrict__ dUce,
                                                  const R* __restrict__ JGe, const R* __restrict__ iJe,
                                                  R* __restrict__ divFe, R* __restrict__ disFne, VFace<R> vf,
                                                  Tabs tb, Phys ph) {
-  volume_body<D, P, VISC, R>(E, Ue, Us, dUce, JGe, iJe, divFe, disFne, vf, tb, ph);
+  volume_body<D, P, VISC, R, NT>(E, Ue, Us, dUce, JGe, iJe, divFe, disFne, vf, tb, ph);
 }
-template <int D, int P, bool VISC, class R>
+template <int D, int P, bool VISC, class R, bool NT>
 __global__ __launch_bounds__(kMaxBlock) __attribute__((amdgpu_waves_per_eu(4))) void k_volume_w4(
     int E, const double* __restrict__ Ue, const R* __restrict__ Us, const R* __restrict__ dUce, const R* __restrict__ JGe,
     const R* __restrict__ iJe, R* __restrict__ divFe, R* __restrict__ disFne, VFace<R> vf, Tabs tb,
     Phys ph) {
-  volume_body<D, P, VISC, R>(E, Ue, Us, dUce, JGe, iJe, divFe, disFne, vf, tb, ph);
+  volume_body<D, P, VISC, R, NT>(E, Ue, Us, dUce, JGe, iJe, divFe, disFne, vf, tb, ph);
 }
 
 // ---------------------------------------------------------------------------
@@ -679,7 +697,7 @@ struct RkB {
 // Arrays that a kernel reads or writes exactly once per launch go through
 // nontemporal loads and stores (they stream past the L2); arrays read twice
 // (the metric terms, the LDG side parts) stay cached.
-template <int D, int P, class R>
+template <int D, int P, class R, bool NT>
 __global__ __launch_bounds__(kMaxBlock) void k_rk(int E, int mode, double a, double bcoef, const R* __restrict__ divFe,
                                              const R* __restrict__ Fne, const R* __restrict__ disFne,
                                              const double* __restrict__ detJe, const double* __restrict__ dtp,
@@ -713,13 +731,13 @@ __global__ __launch_bounds__(kMaxBlock) void k_rk(int E, int mode, double a, dou
 #pragma unroll
         for (int k = 0; k < N; k++) {
           const R bp = Bve[pb + k * F];
-          sD[(bb * N + k) * F + f] = (__builtin_nontemporal_load(&Fne[ob + k * F]) +
-                                      __builtin_nontemporal_load(&disFne[ob + k * F])) + (side == 0 ? bp : -bp) * dA;
+          sD[(bb * N + k) * F + f] = (ld_s<NT>(&Fne[ob + k * F]) +
+                                      ld_s<NT>(&disFne[ob + k * F])) + (side == 0 ? bp : -bp) * dA;
         }
       } else {
 #pragma unroll
         for (int k = 0; k < N; k++)
-          sD[(bb * N + k) * F + f] = __builtin_nontemporal_load(&Fne[ob + k * F]) + __builtin_nontemporal_load(&disFne[ob + k * F]);
+          sD[(bb * N + k) * F + f] = ld_s<NT>(&Fne[ob + k * F]) + ld_s<NT>(&disFne[ob + k * F]);
       }
     }
   } else if (Bve) {
@@ -740,14 +758,14 @@ __global__ __launch_bounds__(kMaxBlock) void k_rk(int E, int mode, double a, dou
         R bv = Bve[ob + k * F];
         if (side != 2) bv += Bve[pb + k * F];
         if (side == 1) bv = -bv;
-        sD[(bb * N + k) * F + f] = (__builtin_nontemporal_load(&Fne[ob + k * F]) + bv * dA) -
-                                   __builtin_nontemporal_load(&disFne[ob + k * F]);
+        sD[(bb * N + k) * F + f] = (ld_s<NT>(&Fne[ob + k * F]) + bv * dA) -
+                                   ld_s<NT>(&disFne[ob + k * F]);
       }
     }
   } else {
     const long long o = (long long)e0 * N * F;
     for (int i = threadIdx.x; i < nb * N * F; i += blockDim.x)
-      sD[i] = __builtin_nontemporal_load(&Fne[o + i]) - __builtin_nontemporal_load(&disFne[o + i]);
+      sD[i] = ld_s<NT>(&Fne[o + i]) - ld_s<NT>(&disFne[o + i]);
   }
   __syncthreads();
   if (mode == 5) {
@@ -759,7 +777,7 @@ __global__ __launch_bounds__(kMaxBlock) void k_rk(int E, int mode, double a, dou
         R c = 0;
 #pragma unroll
         for (int j = 0; j < NC; j++) c += tw<R>(tb.cor_w, tb.cor_wf)[s * NC + j] * sD[(b * N + k) * F + tb.cor_idx[s * NC + j]];
-        __builtin_nontemporal_store(double(c + __builtin_nontemporal_load(&divFe[(e * N + k) * S + s])), &divF0e[(e * N + k) * S + s]);
+        st_s<NT>(double(c + ld_s<NT>(&divFe[(e * N + k) * S + s])), &divF0e[(e * N + k) * S + s]);
       }
     }
     return;  // uniform over the block: no barrier follows for anyone
@@ -769,49 +787,49 @@ __global__ __launch_bounds__(kMaxBlock) void k_rk(int E, int mode, double a, dou
     // dt: 0 global, 1 per element (local CFL), 2 per mesh (batched global CFL)
     const double dt = (local == 1) ? eleDt[e] : (local == 2 ? meshDt[eleMesh[e]] : *dtp);
     const bool frozen = active && !active[eleMesh[e]];
-    const double dj = __builtin_nontemporal_load(&detJe[e * S + s]);
+    const double dj = ld_s<NT>(&detJe[e * S + s]);
 #pragma unroll
     for (int k = 0; k < N; k++) {
       const long long iu = (e * N + k) * S + s;
       R c = 0;
 #pragma unroll
       for (int j = 0; j < NC; j++) c += tw<R>(tb.cor_w, tb.cor_wf)[s * NC + j] * sD[(b * N + k) * F + tb.cor_idx[s * NC + j]];
-      const double r0 = double(c + __builtin_nontemporal_load(&divFe[iu]));
-      if (divF0e) __builtin_nontemporal_store(r0, &divF0e[iu]);
+      const double r0 = double(c + ld_s<NT>(&divFe[iu]));
+      if (divF0e) st_s<NT>(r0, &divF0e[iu]);
       // P-multigrid source: divF_spts[step] + src_spts (src/solver.cpp:480,514)
       const double r = srce ? r0 + srce[iu] : r0;
       // U itself is an input of the first and the single stage only (and of
       // frozen batch members, which keep it)
-      const double u = (mode == 0 || mode == 3 || frozen) ? __builtin_nontemporal_load(&Ue[iu]) : 0.0;
+      const double u = (mode == 0 || mode == 3 || frozen) ? ld_s<NT>(&Ue[iu]) : 0.0;
       double un;
       if (mode == 0) {
-        __builtin_nontemporal_store(u, &U0e[iu]);
-        if (Rst) __builtin_nontemporal_store(R(r), &Rst[iu]);
-        else __builtin_nontemporal_store(u - bcoef * r / dj * dt, &Uacc[iu]);
+        st_s<NT>(u, &U0e[iu]);
+        if (Rst) st_s<NT>(R(r), &Rst[iu]);
+        else st_s<NT>(u - bcoef * r / dj * dt, &Uacc[iu]);
         un = u - a * r / dj * dt;
       } else if (mode == 1) {
-        if (Rst) __builtin_nontemporal_store(R(r), &Rst[stage * nS + iu]);
-        else __builtin_nontemporal_store(__builtin_nontemporal_load(&Uacc[iu]) - bcoef * r / dj * dt, &Uacc[iu]);
-        un = __builtin_nontemporal_load(&U0e[iu]) - a * r / dj * dt;
+        if (Rst) st_s<NT>(R(r), &Rst[stage * nS + iu]);
+        else st_s<NT>(ld_s<NT>(&Uacc[iu]) - bcoef * r / dj * dt, &Uacc[iu]);
+        un = ld_s<NT>(&U0e[iu]) - a * r / dj * dt;
       } else if (mode == 2) {
         double acc;
         if (Rst) {
-          acc = __builtin_nontemporal_load(&U0e[iu]);
-          for (int j = 0; j < stage; j++) acc = acc - bs.b[j] * double(__builtin_nontemporal_load(&Rst[j * nS + iu])) / dj * dt;
+          acc = ld_s<NT>(&U0e[iu]);
+          for (int j = 0; j < stage; j++) acc = acc - bs.b[j] * double(ld_s<NT>(&Rst[j * nS + iu])) / dj * dt;
         } else {
-          acc = __builtin_nontemporal_load(&Uacc[iu]);
+          acc = ld_s<NT>(&Uacc[iu]);
         }
         un = acc - bcoef * r / dj * dt;
       } else if (mode == 3) {
         un = u - bcoef * r / dj * dt;
       } else {
-        un = __builtin_nontemporal_load(&U0e[iu]) - a * r / dj * dt;
+        un = ld_s<NT>(&U0e[iu]) - a * r / dj * dt;
       }
       if (frozen) un = u;
       // with an FP32 stage copy, the FP64 state is written at the step's
       // last stage only (earlier stages leave it at the step's start)
-      if (!Us || fin) __builtin_nontemporal_store(un, &Ue[iu]);
-      if (Us) __builtin_nontemporal_store(R(un), &Us[iu]);
+      if (!Us || fin) st_s<NT>(un, &Ue[iu]);
+      if (Us) st_s<NT>(R(un), &Us[iu]);
       sU[(b * N + k) * S + spt_lds(D, P1, s)] = R(un);
     }
   }
@@ -823,7 +841,7 @@ __global__ __launch_bounds__(kMaxBlock) void k_rk(int E, int mode, double a, dou
       R acc = 0;
 #pragma unroll
       for (int m = 0; m < P1; m++) acc += tw<R>(tb.ext_w, tb.ext_wf)[f * P1 + m] * sU[(bb * N + k) * S + tb.ext_idx[f * P1 + m]];
-      __builtin_nontemporal_store(acc, &Ufe[((long long)(e0 + bb) * N + k) * F + f]);
+      st_s<NT>(acc, &Ufe[((long long)(e0 + bb) * N + k) * F + f]);
     }
   }
 }
@@ -1119,6 +1137,7 @@ bool fused_setup(Ctx& c, const fhip_desc* d) {
                 (const double*)(base + off[9]), (const float*)(base + off[10]), (const float*)(base + off[11]),
                 (const float*)(base + off[12]), (const float*)(base + off[13]), (const float*)(base + off[14])};
   const long long E = z.E, S = z.S, F = z.F, N = z.N, D = z.D;
+  st->stream = E * N * S * (long long)sizeof(double) > kStreamBytes;
   st->Ue = dal<double>(E * N * S);
   st->U0e = dal<double>(E * N * S);
   st->Uacc = dal<double>(E * N * S);
@@ -1309,27 +1328,33 @@ struct Runner {
     Work<R>& w = st.work<R>();
     const int nFF = c.z.nFaceFpts;
     if (nFF == 0) return;
-    hipLaunchKernelGGL((k_face_inv<D, P, R>), dim3(nblk(nFF, nt)), dim3(nt), 0, c.stream, nFF, st.pL, st.pR, st.bc,
-                       w.Ufe, w.nrm, w.dA, w.Fne, st.waveSpe, w.dUce, c.ph, st.bst);
+    if (st.stream)
+      hipLaunchKernelGGL((k_face_inv<D, P, R, true>), dim3(nblk(nFF, nt)), dim3(nt), 0, c.stream, nFF, st.pL, st.pR,
+                         st.bc, w.Ufe, w.nrm, w.dA, w.Fne, st.waveSpe, w.dUce, c.ph, st.bst);
+    else
+      hipLaunchKernelGGL((k_face_inv<D, P, R, false>), dim3(nblk(nFF, nt)), dim3(nt), 0, c.stream, nFF, st.pL, st.pR,
+                         st.bc, w.Ufe, w.nrm, w.dA, w.Fne, st.waveSpe, w.dUce, c.ph, st.bst);
     FCHK(hipGetLastError());
   }
-  template <bool VISC>
+  template <bool VISC, bool NT>
   static void launch_volume_t(Ctx& c, int nt) {
     FusedState& st = fs(c);
     Work<R>& w = st.work<R>();
     const VFace<R> vf{st.fpart, w.nf, w.Bve, w.gBnd, w.dA, st.nBnd};
     // FP64 hexes above P3 need more than 128 VGPRs: no cap there
     if constexpr (D == 2 || P <= 3 || sizeof(R) == 4)
-      hipLaunchKernelGGL((k_volume_w4<D, P, VISC, R>), dim3(nblk(c.z.E, nt / S)), dim3(nt), lds(kVolume, nt, VISC),
-                         c.stream, c.z.E, st.Ue, w.Us, w.dUce, w.JG, w.iJ, w.divFe, w.disFne, vf, st.tb, c.ph);
+      hipLaunchKernelGGL((k_volume_w4<D, P, VISC, R, NT>), dim3(nblk(c.z.E, nt / S)), dim3(nt),
+                         lds(kVolume, nt, VISC), c.stream, c.z.E, st.Ue, w.Us, w.dUce, w.JG, w.iJ, w.divFe, w.disFne,
+                         vf, st.tb, c.ph);
     else
-      hipLaunchKernelGGL((k_volume<D, P, VISC, R>), dim3(nblk(c.z.E, nt / S)), dim3(nt), lds(kVolume, nt, VISC),
+      hipLaunchKernelGGL((k_volume<D, P, VISC, R, NT>), dim3(nblk(c.z.E, nt / S)), dim3(nt), lds(kVolume, nt, VISC),
                          c.stream, c.z.E, st.Ue, w.Us, w.dUce, w.JG, w.iJ, w.divFe, w.disFne, vf, st.tb, c.ph);
     FCHK(hipGetLastError());
   }
   static void launch_volume(Ctx& c, int nt) {
-    if (c.prm.viscous) launch_volume_t<true>(c, nt);
-    else launch_volume_t<false>(c, nt);
+    const bool sm = fs(c).stream;
+    if (c.prm.viscous) sm ? launch_volume_t<true, true>(c, nt) : launch_volume_t<true, false>(c, nt);
+    else sm ? launch_volume_t<false, true>(c, nt) : launch_volume_t<false, false>(c, nt);
   }
   static void launch_face_bnd(Ctx& c, int nt) {
     FusedState& st = fs(c);
@@ -1345,8 +1370,9 @@ struct Runner {
     RkB bs{};
     for (int j = 0; j < c.z.nRK && j < 8; j++) bs.b[j] = c.prm.rk_b[j];
     R* rst = (w.Rst && !st.use_src) ? w.Rst : nullptr;
-    hipLaunchKernelGGL((k_rk<D, P, R>), dim3(nblk(c.z.E, nt / S)), dim3(nt), lds(kRk, nt, false), c.stream, c.z.E, mode,
-                       a, b, w.divFe, w.Fne, w.disFne, st.detJe, c.d_dt, c.eleDt,
+    auto* kern = st.stream ? k_rk<D, P, R, true> : k_rk<D, P, R, false>;
+    hipLaunchKernelGGL(kern, dim3(nblk(c.z.E, nt / S)), dim3(nt), lds(kRk, nt, false), c.stream, c.z.E, mode, a, b,
+                       w.divFe, w.Fne, w.disFne, st.detJe, c.d_dt, c.eleDt,
                        c.prm.dt_type == 2 ? 1 : (c.prm.dt_type == 1 && st.nMesh > 1 ? 2 : 0), st.Ue, st.U0e, st.Uacc,
                        write0 ? st.divF0e : nullptr, w.Ufe, st.tb, st.eleMesh, st.meshDt, st.active,
                        st.use_src ? st.srce : nullptr, rst, stage, bs, st.fpart, w.dA,
