@@ -11,18 +11,23 @@
 //
 // Data live element-major on the device ([ele][field][point]) so that a block
 // of elements is one contiguous slab.  Per RK stage:
-//   face_inv   common inviscid flux at every face flux point (+ LDG dUc)
+//   face_inv   common inviscid flux times dA on both sides of every face
+//              flux point (+ LDG common U, dUc)
 //   volume     [gradient + correction + physical transform], inviscid (and
 //              viscous) flux, transformed flux, divergence, discontinuous
-//              normal flux at the faces [and gradient at the faces]
-//   face_visc  LDG common viscous flux (viscous runs only)
-//   rk         correction of the divergence, RK update, U at the faces of
-//              the next stage
+//              normal flux at the faces [and each side's part of the LDG
+//              viscous flux at its flux points]
+//   face_bnd   LDG viscous flux at boundary flux points (viscous runs with
+//              boundaries only)
+//   rk         correction of the divergence (common flux = inviscid part +
+//              both sides' LDG parts), RK update, U at the faces of the next
+//              stage
 //
 // Every kernel is templated on R, the type of the face and volume
 // intermediates and of the flux arithmetic: double for the FP64 path, float
 // for the mixed path.  The solution, its RK stage copies, the metric terms,
-// the wave speeds, dt and the residual array stay FP64 in both.
+// the wave speeds, dt and the residual array stay FP64 in both (the mixed
+// path also keeps FP32 roundings of the geometry and of U for its kernels).
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
