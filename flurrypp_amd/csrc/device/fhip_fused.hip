@@ -357,6 +357,11 @@ __global__ __launch_bounds__(kMaxBlock) void k_face_inv(int nFF, const int* __re
 template <class R>
 constexpr bool kJumpInVolume = sizeof(R) == 4;
 
+// Shapes whose volume kernel is held to 4 waves per SIMD (128 VGPRs): quads
+// and the FP32 path.  FP64 hexes need more registers and run unheld.
+template <int D, class R>
+constexpr bool kHold = D == 2 || sizeof(R) == 4;
+
 // Viscous runs: what the volume kernel leaves at the flux points in place of
 // the gradient there (see the face loop of volume_body).
 template <class R>
@@ -522,9 +527,11 @@ __device__ __forceinline__ void volume_body(int E, const double* __restrict__ Ue
         for (int k = 0; k < N; k++) st_s<NT>(bv[k], &vf.Bve[(ee * N + k) * F + f]);
       }
     }
-    // the metric terms again (from cache): keeping them live across the face
-    // loop above costs more registers than the hold allows
-    if (act) load_G();
+    // under the register hold, the metric terms again (from cache): keeping
+    // them live across the face loop above costs more registers than it allows
+    if constexpr (kHold<D, R>) {
+      if (act) load_G();
+    }
   }
   // phase 2: flux at the point, transformed to the reference element.  It is
   // formed in registers and stored after a barrier: viscous runs store it over
@@ -1349,7 +1356,7 @@ struct Runner {
     // FP64 hexes need more than 128 VGPRs (P3 spills 19 under the hold): no
     // hold there; unheld they run at 3 waves/SIMD without spills, measured
     // 2% faster at P3 once the streaming arrays bypass the caches
-    if constexpr (D == 2 || sizeof(R) == 4)
+    if constexpr (kHold<D, R>)
       hipLaunchKernelGGL((k_volume_w4<D, P, VISC, R, NT>), dim3(nblk(c.z.E, nt / S)), dim3(nt),
                          lds(kVolume, nt, VISC), c.stream, c.z.E, st.Ue, w.Us, w.dUce, w.JG, w.iJ, w.divFe, w.disFne,
                          vf, st.tb, c.ph);
