@@ -1460,7 +1460,9 @@ struct Runner {
     hipEvent_t e0, e1;
     FCHK(hipEventCreate(&e0));
     FCHK(hipEventCreate(&e1));
-    const int cand[4] = {64, 128, 256, 512};
+    // multiples of the 64-wide wavefront; element kernels hold floor(NT/S)
+    // elements, so sizes between the powers of two pay off where S divides them
+    const int cand[7] = {64, 128, 192, 256, 320, 384, 512};
     for (int k = kExtrap; k <= kRk; k++) {
       if (k == kFaceBnd && (!visc || st.nBnd == 0)) continue;
       auto run = [&](int nt) {

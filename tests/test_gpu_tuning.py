@@ -44,7 +44,7 @@ def test_block_sizes_do_not_change_results(name, path):
     probe = Solver(setup.params, setup=setup)
     tried = 0
     for k in range(5):
-        for nt in (64, 128, 512):
+        for nt in (64, 128, 192, 320, 512):
             sizes = [256] * 5
             sizes[k] = nt
             try:
