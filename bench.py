@@ -156,6 +156,31 @@ def tgv_mixed_form(n: int, order: int, steps: int, warmup: int, tune: bool) -> d
     }
 
 
+def vortex2d_form(steps: int, tune: bool) -> dict:
+    """The 2-D Euler isentropic vortex at its full size (32x32 jittered
+    periodic quads, P3, Rusanov, RK44, CFL 0.5) on the FP64 fused path: a
+    launch-bound case (16,384 DOF)."""
+    from flurrypp_amd import cases
+    from flurrypp_amd._native import PATH_FUSED
+    from flurrypp_amd.solver import Solver
+    setup, U = cases.vortex2d(n=32, order=3)
+    s = Solver(setup.params, setup=setup)
+    s.initializeSolution(U)
+    if tune:
+        s.autotune(PATH_FUSED)
+    s.update(10, path=PATH_FUSED)
+    s.sync()
+    t0 = time.perf_counter()
+    s.update(steps, path=PATH_FUSED, sync=False)
+    s.sync()
+    el = time.perf_counter() - t0
+    s.close()
+    dof, nRK = setup.nEles * setup.nSpts, setup.params.nRKSteps
+    return {"workload": "2D Euler isentropic vortex, 32x32 jittered periodic quads, P3, Rusanov, RK44, CFL 0.5",
+            "dof": dof, "steps": steps, "value": round(dof * nRK * steps / el / 1e9, 4), "unit": "GDOF·stage/s",
+            "ms_per_step": round(el / steps * 1e3, 4)}
+
+
 def kinetic_energy(setup, U) -> float:
     """E_k = (1/|Omega|) sum w detJ rho |u|^2 / 2 over solution points (FP64)."""
     D = setup.nDims
@@ -312,6 +337,8 @@ def main() -> int:
     s.copyUspts_U0()
     s.sync()
     s.close()
+    if a.path == "fused":
+        forms["vortex2d_32"] = vortex2d_form(100, not a.no_tune)
     if a.path == "fused" and not a.no_mixed and a.mixed_n > 0:
         log(f"bench: mixed-precision form at {a.mixed_n}^3")
         forms[f"mixed_{a.mixed_n}"] = tgv_mixed_form(a.mixed_n, a.order, a.steps, a.warmup, not a.no_tune)

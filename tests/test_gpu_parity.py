@@ -228,3 +228,24 @@ def test_multigrid_source_term():
         x.calcResidual(1)
         x.timeStepB(1, src)
     assert rel_l2(s.get_U(), o.get_U()) < 1e-14
+
+
+@pytest.mark.parametrize("path", [PATH_STAGED, PATH_FUSED])
+def test_vortex2d_full_size_100_steps(path):
+    """The 2-D Euler vortex at its full configuration (32x32 jittered periodic
+    quads, P3, Rusanov, RK44, global CFL 0.5) for 100 steps against the
+    oracle: state within 1e-12 relative L2, residual norms within round-off."""
+    setup, U = cases.vortex2d(n=32, order=3)
+    p = setup.params
+    print(f"vortex2d 32x32 P3 path {path}", flush=True)
+    s = Solver(p, setup=setup)
+    o = Oracle.from_setup(setup)
+    s.initializeSolution(U)
+    o.initializeSolution(U, p.dtType)
+    s.update(100, path=path)
+    o.update(100)
+    err = rel_l2(s.get_U(), o.get_U())
+    print(f"vortex2d 32x32 P3 path={path} U rel_l2 after 100 steps = {err:.3e}", flush=True)
+    assert err < TOL_TRAJ  # a state this close after 100 steps also means the same dt at every step
+    np.testing.assert_allclose(s.residual_norm(2), o.residual_norm(2), rtol=1e-10, atol=0)
+    s.close()
