@@ -181,6 +181,109 @@ def vortex2d_form(steps: int, tune: bool) -> dict:
             "ms_per_step": round(el / steps * 1e3, 4)}
 
 
+def dense_operator_form(n: int, order: int, reps: int = 5) -> dict:
+    """The operator applications of one Navier-Stokes RK stage as the reference
+    forms them: dense GEMMs over every element (src/solver.cpp:556-1158,
+    extrapolateU, calcGradU_spts, correctGradU, extrapolateGradU,
+    extrapolateNormalFlux, calcDivF_spts, correctDivFlux), on the hex P3
+    Taylor-Green shapes, through rocBLAS on the matrix cores: in FP64, and in
+    BF16 with FP32 accumulation as hi+lo split pairs.  GEMMs only (no flux,
+    face or RK work), to set beside the fused path's whole stage."""
+    import ctypes as C
+    from flurrypp_amd.mesh import operators
+    hip = C.CDLL("libamdhip64.so")
+    rb = C.CDLL("/opt/rocm/lib/librocblas.so")
+
+    def ok(rc, what):
+        if rc != 0:
+            raise RuntimeError(f"{what} failed ({rc})")
+
+    op = operators(3, order)
+    F, S = op.spts_to_fpts.shape
+    N = 5
+    cols = n ** 3 * N
+    # (operator, count): A is rows x k in row-major, applied to a k x cols block
+    ops = [(op.spts_to_fpts, 1), (op.grad_spts[0], 3), (op.correct_u[0], 3), (op.spts_to_fpts, 3),
+           (op.extrap_fn[0], 3), (op.grad_spts[0], 3), (op.correction, 1)]
+    flop = sum(2.0 * a.shape[0] * a.shape[1] * cols * c for a, c in ops)
+    handle = C.c_void_p()
+    ok(rb.rocblas_create_handle(C.byref(handle)), "rocblas_create_handle")
+    bufs = []
+
+    def dmalloc(nbytes, fill):
+        p = C.c_void_p()
+        ok(hip.hipMalloc(C.byref(p), C.c_size_t(nbytes)), "hipMalloc")
+        ok(hip.hipMemset(p, C.c_int(fill), C.c_size_t(nbytes)), "hipMemset")
+        bufs.append(p)
+        return p
+
+    def upload(arr):
+        arr = np.ascontiguousarray(arr)
+        p = dmalloc(arr.nbytes, 0)
+        ok(hip.hipMemcpy(p, arr.ctypes.data_as(C.c_void_p), C.c_size_t(arr.nbytes), C.c_int(1)), "hipMemcpy")
+        return p
+
+    def timed(run):
+        run()
+        ok(hip.hipDeviceSynchronize(), "sync")
+        ev0, ev1 = C.c_void_p(), C.c_void_p()
+        hip.hipEventCreate(C.byref(ev0))
+        hip.hipEventCreate(C.byref(ev1))
+        hip.hipEventRecord(ev0, None)
+        for _ in range(reps):
+            run()
+        hip.hipEventRecord(ev1, None)
+        hip.hipEventSynchronize(ev1)
+        ms = C.c_float()
+        hip.hipEventElapsedTime(C.byref(ms), ev0, ev1)
+        return ms.value / reps
+
+    out = {"workload": f"operator GEMMs of one NS RK stage, {n}^3 hex P{order} ({cols:,} columns), rocBLAS",
+           "gemms_per_stage": sum(c for _, c in ops), "gflop_per_stage": round(flop / 1e9, 2)}
+    try:
+        # FP64: row-major C[m x cols] = A[m x k] B[k x cols] is column-major C^T = B^T A^T
+        big = max(F, S) * cols * 8
+        B64, C64 = dmalloc(big, 0x3F), dmalloc(big, 0)
+        A64 = [(upload(a.astype(np.float64)), a.shape, c) for a, c in ops]
+        one, zero = C.c_double(1.0), C.c_double(0.0)
+
+        def run64():
+            for pa, (m, k), c in A64:
+                for _ in range(c):
+                    ok(rb.rocblas_dgemm(handle, 111, 111, cols, m, k, C.byref(one), B64, cols, pa, k,
+                                        C.byref(zero), C64, cols), "rocblas_dgemm")
+        t64 = timed(run64)
+        out["fp64"] = {"ms_per_stage": round(t64, 4), "TFLOP/s": round(flop / t64 / 1e9, 2),
+                       "frac_of_fp64_matrix_peak": round(flop / t64 / 1e9 / FP64_PEAK_TFS, 4)}
+        # BF16 inputs (hi and lo parts), FP32 accumulate and output
+        Bb, Cf = dmalloc(max(F, S) * cols * 2, 0x3F), dmalloc(max(F, S) * cols * 4, 0)
+
+        def split(a):
+            # hi = a rounded down to BF16's 8 mantissa bits, lo = the rest (BF16 bit patterns)
+            a32 = np.ascontiguousarray(a, dtype=np.float32)
+            hi = (a32.view(np.uint32) & np.uint32(0xFFFF0000)).view(np.float32)
+            lo = (a32 - hi).astype(np.float32)
+            return [(x.view(np.uint32) >> 16).astype(np.uint16) for x in (hi, lo)]
+        Abf = [(upload(h), upload(lo), a.shape, c) for a, c in ops for h, lo in [split(a)]]
+        f1, f0 = C.c_float(1.0), C.c_float(0.0)
+
+        def runbf():
+            for ph, pl, (m, k), c in Abf:
+                for _ in range(c):
+                    for pa, beta in ((ph, f0), (pl, f1)):
+                        ok(rb.rocblas_gemm_ex(handle, 111, 111, cols, m, k, C.byref(f1), Bb, 168, cols, pa, 168, k,
+                                              C.byref(beta), Cf, 151, cols, Cf, 151, cols, 151, 0, 0, 0),
+                           "rocblas_gemm_ex")
+        tbf = timed(runbf)
+        out["bf16_hi_lo"] = {"ms_per_stage": round(tbf, 4), "gemms": 2 * out["gemms_per_stage"],
+                             "TFLOP/s": round(2 * flop / tbf / 1e9, 2)}
+    finally:
+        for p in bufs:
+            hip.hipFree(p)
+        rb.rocblas_destroy_handle(handle)
+    return out
+
+
 def kinetic_energy(setup, U) -> float:
     """E_k = (1/|Omega|) sum w detJ rho |u|^2 / 2 over solution points (FP64)."""
     D = setup.nDims
@@ -217,6 +320,7 @@ def main() -> int:
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-mixed", action="store_true")
     ap.add_argument("--no-tune", action="store_true", help="keep the default block sizes (256)")
+    ap.add_argument("--no-dense", action="store_true", help="skip timing the dense operator GEMMs")
     ap.add_argument("--mixed-n", type=int, default=64,
                     help="elements per direction of the large mixed-precision Taylor-Green form (0 skips it)")
     ap.add_argument("--batch-cases", type=int, default=64, help="0 skips the batched form")
@@ -339,6 +443,12 @@ def main() -> int:
     s.close()
     if a.path == "fused":
         forms["vortex2d_32"] = vortex2d_form(100, not a.no_tune)
+        if visc and not a.no_dense:
+            log("bench: dense operator GEMMs (rocBLAS)")
+            dense = dense_operator_form(a.n, a.order)
+            dense["fused_fp64_stage_ms"] = round(sum(kernels[k]["avg_ms"] for k in kernels
+                                                     if k in work and k != "fused_extrapolate"), 4)
+            forms["dense_operators"] = dense
     if a.path == "fused" and not a.no_mixed and a.mixed_n > 0:
         log(f"bench: mixed-precision form at {a.mixed_n}^3")
         forms[f"mixed_{a.mixed_n}"] = tgv_mixed_form(a.mixed_n, a.order, a.steps, a.warmup, not a.no_tune)
