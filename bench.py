@@ -441,23 +441,34 @@ def main() -> int:
     s.copyUspts_U0()
     s.sync()
     s.close()
+    def optional(name, fn):
+        # a side form that fails (a missing library, say) must not cost the headline line
+        try:
+            return fn()
+        except Exception as ex:
+            log(f"bench: {name} form failed: {ex!r}")
+            return {"error": repr(ex)}
+
     if a.path == "fused":
-        forms["vortex2d_32"] = vortex2d_form(100, not a.no_tune)
+        forms["vortex2d_32"] = optional("vortex2d_32", lambda: vortex2d_form(100, not a.no_tune))
         if visc and not a.no_dense:
             log("bench: dense operator GEMMs (rocBLAS)")
-            dense = dense_operator_form(a.n, a.order)
-            dense["fused_fp64_stage_ms"] = round(sum(kernels[k]["avg_ms"] for k in kernels
-                                                     if k in work and k != "fused_extrapolate"), 4)
+            dense = optional("dense_operators", lambda: dense_operator_form(a.n, a.order))
+            if "error" not in dense:
+                dense["fused_fp64_stage_ms"] = round(sum(kernels[k]["avg_ms"] for k in kernels
+                                                         if k in work and k != "fused_extrapolate"), 4)
             forms["dense_operators"] = dense
     if a.path == "fused" and not a.no_mixed and a.mixed_n > 0:
         log(f"bench: mixed-precision form at {a.mixed_n}^3")
-        forms[f"mixed_{a.mixed_n}"] = tgv_mixed_form(a.mixed_n, a.order, a.steps, a.warmup, not a.no_tune)
+        forms[f"mixed_{a.mixed_n}"] = optional(f"mixed_{a.mixed_n}", lambda: tgv_mixed_form(
+            a.mixed_n, a.order, a.steps, a.warmup, not a.no_tune))
     if a.batch_cases > 0:
         log(f"bench: batched form ({a.batch_cases} cases)")
-        forms["batched"] = batched_form(a.batch_cases, 11, a.batch_tol, a.batch_max_steps, a.batch_nx, a.batch_ny)
+        forms["batched"] = optional("batched", lambda: batched_form(a.batch_cases, 11, a.batch_tol,
+                                                                     a.batch_max_steps, a.batch_nx, a.batch_ny))
     if a.serve_requests > 0:
         log(f"bench: served form ({a.serve_requests} requests)")
-        forms["served"] = served_form(a.serve_requests, a.serve_rate, 13)
+        forms["served"] = optional("served", lambda: served_form(a.serve_requests, a.serve_rate, 13))
 
     cpu = None
     if not a.no_cpu:
