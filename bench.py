@@ -333,9 +333,8 @@ def main() -> int:
     a = ap.parse_args()
 
     if a.gpus != 1:
-        print(f"bench.py runs on one GPU only; --gpus {a.gpus} is not supported: this build measures one "
-              "GPU's share of the path (the mesh would be split by element partition, one context per GPU).",
-              file=sys.stderr)
+        print(f"bench.py runs on one GPU only; --gpus {a.gpus} is not supported: it measures one GPU's share "
+              "of the path (a mesh shards by element partition, one context per GPU).", file=sys.stderr)
         return 2
 
     from flurrypp_amd import cases, perfmodel
