@@ -14,8 +14,8 @@
 * CSV (`writeCSV`, src/output.cpp:60-135): x, y, z, then rho, u, v, p at the
   solution points and the flux points of every element.
 
-Euler/Navier-Stokes fields only (the advection-diffusion branch is outside
-this build).
+Euler/Navier-Stokes fields only (the reference's advection-diffusion branch
+is not written).
 """
 from __future__ import annotations
 
