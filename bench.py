@@ -315,8 +315,8 @@ def main() -> int:
     ap.add_argument("--order", type=int, default=3)
     ap.add_argument("--inviscid", action="store_true")
     ap.add_argument("--path", choices=["fused", "staged"], default="fused")
-    ap.add_argument("--cpu-n", type=int, default=12)
-    ap.add_argument("--cpu-steps", type=int, default=8)
+    ap.add_argument("--cpu-n", type=int, default=16)
+    ap.add_argument("--cpu-steps", type=int, default=10)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-mixed", action="store_true")
     ap.add_argument("--no-tune", action="store_true", help="keep the default block sizes (256)")
