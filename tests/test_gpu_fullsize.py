@@ -5,11 +5,13 @@ properties that hold at any size:
 * conservation: on a periodic mesh the FR scheme with the DG correction
   conserves the integral of every conserved variable (sum of w_s detJ U over
   all solution points): momentum to 1e-12 of its magnitude.  Mass and energy
-  drift more in the reference's own scheme (its 15-digit Gauss constants make
-  the discrete identities hold to ~1e-15 per face only), growing with the
-  mesh: in 5 steps the oracle drifts by 3e-14, 7e-13, 2.4e-12 of the mass and
-  2e-14, 1.2e-13, 1.5e-13 of the energy at 6^3, 12^3, 16^3, so their bars at
-  48^3 are 1e-9 and 1e-10 (a side-dependent common flux would show at ~1e-3);
+  drift more in the reference's own scheme, growing with the mesh: each
+  element's residual is a sum of O(|F|/h) terms rounded at ~1e-16 whose
+  roundings do not cancel across elements (the operators' discrete
+  conservation identities themselves hold to 7e-16).  In 5 steps the oracle
+  drifts by 3e-14, 7e-13, 2.4e-12 of the mass and 2e-14, 1.2e-13, 1.5e-13 of
+  the energy at 6^3, 12^3, 16^3, so their bars at 48^3 are 1e-9 and 1e-10 (a
+  side-dependent common flux would show at ~1e-3);
 * the fused path (sparse tensor-product kernels) and the staged path (the
   reference's dense operator sequence), each pinned to the oracle at small
   sizes, agree to 1e-12 relative L2;
