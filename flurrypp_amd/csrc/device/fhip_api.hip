@@ -261,7 +261,7 @@ int fhip_create(const fhip_desc* d, fhip_ctx** out) {
       c.iter = p.init_iter;
       c.batched = d->n_meshes > 1 || d->n_bnd_states > 0;
       c.fused_ok = fhip::fused_setup(c, d);
-      require(c.fused_ok || !c.batched, "batched contexts need the fused path (quads/hexes, P 1..5)");
+      require(c.fused_ok || !c.batched, "batched contexts need the fused path (quads/hexes, P 0..5)");
       HCHK(hipStreamSynchronize(c.stream));
     } catch (...) {
       destroy_ctx(c);

@@ -398,8 +398,8 @@ __device__ __forceinline__ void volume_body(int E, const double* __restrict__ Ue
   const int sw = spt_lds(D, P1, s);
   const bool act = b < nb;
   const long long e = e0 + b;
-  // the point's metric terms: inviscid runs load them while the slab is
-  // staged; viscous runs (short of registers) load them where they are used
+  // the point's metric terms, loaded first so that their latency overlaps the
+  // staging of the slab (and the gradient of viscous runs)
   R G[D][D] = {};
   auto load_G = [&] {
 #pragma unroll
@@ -407,7 +407,6 @@ __device__ __forceinline__ void volume_body(int E, const double* __restrict__ Ue
 #pragma unroll
       for (int x = 0; x < D; x++) G[r][x] = JGe[((e * D + r) * D + x) * S + s];
   };
-  // metric loads in flight while the slab is staged and the gradient formed
   R inv = 0;
   if (act) {
     load_G();
@@ -628,9 +627,10 @@ __device__ __forceinline__ void volume_body(int E, const double* __restrict__ Ue
 }
 
 // The volume kernel, and the same kernel held to 4 waves per SIMD (at most
-// 128 VGPRs) for quads and the FP32 path, which fit there without spilling:
-// its LDS slabs limit it to 4 waves per SIMD anyway, and the hold lets the
-// metric terms stay in registers across the phases.
+// 128 VGPRs) for quads and the FP32 path, which fit there without spilling
+// (its LDS slabs limit it to 4 waves per SIMD anyway).  Under the hold,
+// viscous runs reload the metric terms after the face loop; inviscid runs
+// keep them in registers throughout.
 template <int D, int P, bool VISC, class R, bool NT>
 __global__ __launch_bounds__(kMaxBlock) void k_volume(int E, const double* __restrict__ Ue, const R* __restrict__ Us,
                                                  const R* __restrict__ dUce,
@@ -706,9 +706,9 @@ __global__ __launch_bounds__(kMaxBlock) void k_face_bnd(int nB, const int* __res
 struct RkB {
   double b[8];
 };
-// Arrays that a kernel reads or writes exactly once per launch go through
-// nontemporal loads and stores (they stream past the L2); arrays read twice
-// (the metric terms, the LDG side parts) stay cached.
+// With NT, the arrays this kernel reads or writes exactly once per launch go
+// through nontemporal loads and stores (they stream past the caches); arrays
+// read twice (the LDG side parts, read by both sides of a face) stay cached.
 template <int D, int P, class R, bool NT>
 __global__ __launch_bounds__(kMaxBlock) void k_rk(int E, int mode, double a, double bcoef, const R* __restrict__ divFe,
                                              const R* __restrict__ Fne, const R* __restrict__ disFne,
@@ -1238,7 +1238,10 @@ bool fused_setup(Ctx& c, const fhip_desc* d) {
         for (long long x = 0; x < D; x++) nl[(e * D + x) * F + f] = d->norm_fpts[(fL * E + eL) * D + x];
       };
       for (int t = 0; t < z.nFaceFpts; t++) {
-        if (part[pl[t]] != -1 || (pr[t] >= 0 && part[pr[t]] != -1)) return false;
+        if (part[pl[t]] != -1 || (pr[t] >= 0 && part[pr[t]] != -1)) {
+          fused_release(c);
+          return false;
+        }
         put_normal(pl[t], pl[t]);
         if (pr[t] < 0) {
           bt.push_back(t);
@@ -1250,7 +1253,10 @@ bool fused_setup(Ctx& c, const fhip_desc* d) {
         }
       }
       for (int q : part)
-        if (q == -1) return false;
+        if (q == -1) {
+          fused_release(c);
+          return false;
+        }
       st->nBnd = (int)bt.size();
       st->fpart = dal<int>(part.size());
       st->bndT = dal<int>(bt.size());
@@ -1507,6 +1513,18 @@ struct Runner {
       if (!valid(k, nt[k], visc)) return false;
     return true;
   }
+  // Each block size brought within its kernel's limits: the default of 256
+  // threads exceeds kMaxLds for the RK kernel and the viscous volume kernel
+  // of P0 hexes (256 elements of 7 FP64 N-vectors per block)
+  static void fit(Ctx& c) {
+    const bool visc = c.prm.viscous != 0;
+    for (int k = kExtrap; k <= kRk; k++)
+      while (!valid(k, blk(c, k), visc) && blk(c, k) > 64) blk(c, k) -= 64;
+  }
+  static void ensure(Ctx& c) {
+    ensure_work<R>(c);
+    fit(c);
+  }
 };
 
 struct Ops {
@@ -1518,12 +1536,13 @@ struct Ops {
   void (*ensure)(Ctx&);
   void (*tune)(Ctx&);
   bool (*check)(Ctx&, const int*);
+  void (*fit)(Ctx&);
 };
 
 template <int D, int P, class R>
 static Ops ops_for() {
   using Rn = Runner<D, P, R>;
-  return Ops{Rn::extrap, Rn::wave, Rn::dt, Rn::residual, Rn::rk, ensure_work<R>, Rn::tune, Rn::check};
+  return Ops{Rn::extrap, Rn::wave, Rn::dt, Rn::residual, Rn::rk, Rn::ensure, Rn::tune, Rn::check, Rn::fit};
 }
 
 template <class R>
@@ -1593,6 +1612,7 @@ void fused_set_blocks(Ctx& c, bool mixed, const int* nt) {
 }
 
 void fused_get_blocks(Ctx& c, bool mixed, int* out) {
+  get_ops(c, mixed).fit(c);
   const int* nt = fs(c).nt[mixed ? 1 : 0];
   std::copy(nt, nt + 5, out);
 }

@@ -196,7 +196,8 @@ int fhip_set_option(fhip_ctx* ctx, int option, int value);
  * the context's current data, keeps the fastest and returns them in
  * block_sizes (may be NULL); the state is left unchanged.  Results are cached
  * for the process per (nDims, order, precision, viscous, log2 nEles).  Block
- * shapes never change results.  Default 256 each. */
+ * shapes never change results.  Default 256 each, or the largest multiple
+ * of 64 below it whose LDS slabs fit 64 KiB (P0 hexes). */
 int fhip_autotune(fhip_ctx* ctx, int path, int* block_sizes);
 int fhip_set_block_sizes(fhip_ctx* ctx, int path, const int* block_sizes);
 int fhip_get_block_sizes(fhip_ctx* ctx, int path, int* block_sizes);

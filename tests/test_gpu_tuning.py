@@ -73,3 +73,20 @@ def test_invalid_block_size_rejected():
     with pytest.raises(Exception):
         s.set_block_sizes([256, 256, 32, 256, 256])   # 32 threads < 36 points per element
     s.close()
+
+
+@pytest.mark.parametrize("path", [PATH_FUSED, PATH_MIXED])
+def test_default_sizes_fit_p0_hexes(path):
+    # 256 threads of P0 hexes (one point per element) would need 70 KiB of LDS
+    # in the FP64 RK and viscous volume kernels: the defaults come down to fit
+    make = lambda: cases.tgv3d(n=4, order=0, viscous=1, jitter=0.1)
+    setup, U = make()
+    s = Solver(setup.params, setup=setup)
+    sizes = s.block_sizes(path)
+    s.set_block_sizes(sizes, path)  # raises unless every size is valid
+    s.close()
+    if path == PATH_FUSED:
+        assert sizes[2] <= 192 and sizes[4] <= 192, sizes
+    ref, _ = _run(make, path)
+    got, _ = _run(make, path, sizes=[128] * 5)
+    assert np.array_equal(got, ref)
