@@ -628,9 +628,10 @@ __device__ __forceinline__ void volume_body(int E, const double* __restrict__ Ue
 
 // The volume kernel, and the same kernel held to 4 waves per SIMD (at most
 // 128 VGPRs) for quads and the FP32 path, which fit there without spilling
-// (its LDS slabs limit it to 4 waves per SIMD anyway).  Under the hold,
-// viscous runs reload the metric terms after the face loop; inviscid runs
-// keep them in registers throughout.
+// (holding the FP32 viscous hex kernel to 5 or 6 waves spills 26 or 41 VGPRs
+// and was measured 13-16% slower).  Under the hold, viscous runs reload the
+// metric terms after the face loop; inviscid runs keep them in registers
+// throughout.
 template <int D, int P, bool VISC, class R, bool NT>
 __global__ __launch_bounds__(kMaxBlock) void k_volume(int E, const double* __restrict__ Ue, const R* __restrict__ Us,
                                                  const R* __restrict__ dUce,
