@@ -7,7 +7,6 @@ immediately (there is no fallback implementation).
 from __future__ import annotations
 
 import ctypes as C
-import os
 from pathlib import Path
 
 import numpy as np

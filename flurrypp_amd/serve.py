@@ -12,7 +12,6 @@ unsteady (a fixed number of steps), in FP64 or mixed precision.
 from __future__ import annotations
 
 import ctypes as C
-import heapq
 import time
 from dataclasses import dataclass, field
 

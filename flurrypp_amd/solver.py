@@ -17,7 +17,7 @@ import numpy as np
 
 from . import _native as nat
 from .input import InputParams
-from .mesh import Mesh, bc_number, operators, pmg_operators, sizes
+from .mesh import Mesh, operators, pmg_operators, sizes
 
 PI = 4.0 * math.atan(1.0)  # include/global.hpp:72
 
