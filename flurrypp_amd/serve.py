@@ -78,6 +78,13 @@ class Server:
     active: list = field(default_factory=list)
     done: list = field(default_factory=list)
 
+    def _room(self, r: Request) -> int:
+        """Steps request r may take before its next residual check (steady) or
+        its last step (unsteady)."""
+        if r.steady_tol is None:
+            return max(0, r.n_steps - r.steps)
+        return max(0, min(r.max_steps - r.steps, self.check_every - r.steps % self.check_every))
+
     def run(self, requests: list[Request]) -> dict:
         pending = sorted(requests, key=lambda r: r.arrival)
         waiting: list[Request] = []
@@ -113,25 +120,31 @@ class Server:
                 if pending:
                     time.sleep(max(0.0, min(0.01, pending[0].arrival - now())))
                 continue
-            # one tick: every active request advances together
+            # one tick: the active requests of each precision advance together,
+            # by at most steps_per_tick and never past a member's next residual
+            # check or its last step
             lib = nat.hip_lib()
+            taken = {}
             for path in (nat.PATH_FUSED, nat.PATH_MIXED):
                 group = [r for r in self.active if (r.precision == "mixed") == (path == nat.PATH_MIXED)]
                 if not group:
                     continue
+                k = min([self.steps_per_tick] + [self._room(r) for r in group])
                 arr = (C.c_void_p * len(group))(*[ctx[r.rid][0]._ctx.value for r in group])
                 cyc = [ctx[r.rid][3] for r in group if ctx[r.rid][3] is not None]
-                if not cyc:
-                    nat.hip_check(lib.fhip_update_many(arr, len(group), self.steps_per_tick, path))
-                    continue
-                for _ in range(self.steps_per_tick):
-                    nat.hip_check(lib.fhip_update_many(arr, len(group), 1, path))
-                    for mg in cyc:
-                        mg.cycle(path, sync=False)
+                if k > 0 and not cyc:
+                    nat.hip_check(lib.fhip_update_many(arr, len(group), k, path))
+                elif k > 0:
+                    for _ in range(k):
+                        nat.hip_check(lib.fhip_update_many(arr, len(group), 1, path))
+                        for mg in cyc:
+                            mg.cycle(path, sync=False)
+                for r in group:
+                    taken[r.rid] = k
             for r in self.active:
                 ctx[r.rid][0].sync()
-                r.steps += self.steps_per_tick
-                total_dof_stages += r.dof * ctx[r.rid][2] * self.steps_per_tick
+                r.steps += taken[r.rid]
+                total_dof_stages += r.dof * ctx[r.rid][2] * taken[r.rid]
             # completion and eviction
             still = []
             for r in self.active:

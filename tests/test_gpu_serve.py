@@ -39,3 +39,26 @@ def test_admission_respects_budget():
     srv.run(reqs)
     spans = sorted((r.admitted, r.finished) for r in srv.done)
     assert all(spans[i + 1][0] >= spans[i][1] - 1e-9 for i in range(len(spans) - 1))
+
+
+def test_steps_exact_when_ticks_do_not_divide_them():
+    # 13 steps at 5 per tick: the last tick takes 3, and the result equals 13
+    # updates of the request alone; a steady request is checked at multiples of
+    # check_every although ticks are 7 steps
+    from flurrypp_amd._native import PATH_FUSED
+    from flurrypp_amd.serve import build_case
+    from flurrypp_amd.solver import Solver
+    un = Request(0, "vortex3d", (3,), 2, 0.0, n_steps=13)
+    st = Request(1, "bump2d", (8, 4), 1, 0.0, steady_tol=1e-30, max_steps=25)
+    srv = Server(max_active=2, steps_per_tick=7, check_every=10, keep_results=True)
+    srv.run([copy.deepcopy(un), copy.deepcopy(st)])
+    got = {r.rid: r for r in srv.done}
+    assert got[0].steps == 13 and got[0].status == "done"
+    assert got[1].steps == 25 and got[1].status == "max_steps"
+    setup, U = build_case(un)
+    s = Solver(setup.params, setup=setup)
+    s.initializeSolution(U)
+    s.update(13, path=PATH_FUSED)
+    ref = s.get_U()
+    s.close()
+    assert np.array_equal(got[0].result, ref)
