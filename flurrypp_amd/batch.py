@@ -267,9 +267,10 @@ class BatchedSolver:
         work = 0
         t0 = time.perf_counter()
         while done < max_steps and not (self.converged | diverged).all():
-            self.step(check_every)
-            done += check_every
-            work += int(dof[~(self.converged | diverged)].sum()) * check_every
+            k = min(check_every, max_steps - done)  # never past max_steps
+            self.step(k)
+            done += k
+            work += int(dof[~(self.converged | diverged)].sum()) * k
             res = self.residuals(res_type)
             now = time.perf_counter() - t0
             for i, r in enumerate(res):

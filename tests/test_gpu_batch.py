@@ -97,3 +97,18 @@ def test_batch_pmg_steady_converges():
     print("pmg steady steps", out["steps"], [r[0] for r in out["residual"]])
     assert out["converged"].all(), out
     b.close()
+
+
+def test_steady_stops_at_max_steps():
+    # max_steps not a multiple of check_every: the last chunk is shortened
+    ms = members()[:3]
+    b = BatchedSolver(ms)
+    out = b.solve_steady(tol=1e-30, max_steps=130, check_every=50)
+    assert list(out["steps"]) == [130] * len(ms), out["steps"]
+    got = b.states()
+    b.close()
+    s = Solver(ms[0].setup.params, setup=ms[0].setup)
+    s.initializeSolution(ms[0].U)
+    s.update(130, path=PATH_FUSED)
+    assert np.array_equal(got[0], s.get_U())
+    s.close()
