@@ -392,7 +392,9 @@ class MultiGrid:
 
 def run(params: InputParams, log=print, output_dir: str | None = ".") -> Solver:
     """The main loop of src/flurry.cpp:120-155 (residual monitor and plot
-    files included; output_dir=None writes no files)."""
+    files included; output_dir=None writes no files).  The error monitor
+    (writeError / writeAllError against a test case's exact solution) is
+    output outside the residual + RK path and is not reproduced."""
     s = Solver(params)
     s.initializeSolution()
     p = params
@@ -407,7 +409,7 @@ def run(params: InputParams, log=print, output_dir: str | None = ".") -> Solver:
         it = s.iter
         if output_dir is not None and (it % p.plotFreq == 0 or it == p.iterMax or p.time >= p.maxTime):
             s.writeData(output_dir)
-        if it % p.monitorResFreq == 0 or it == p.initIter + 1:
+        if it % p.monitorResFreq == 0 or it == p.initIter + 1 or p.time >= p.maxTime:
             dt = s.dt
             if dt < 1e-13:
                 raise RuntimeError("Instability detected - dt approaching zero!")
