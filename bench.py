@@ -13,12 +13,14 @@ fhip_update on the device, with every kernel bracketed by HIP events on the
 context's stream (no synchronisation inside the region); the per-kernel
 averages give the roofline fraction of the dominant kernel.  The CPU
 baseline times the C restatement in oracle/ (serial) on a smaller Taylor-Green
-mesh on this host.
+mesh on this host, on one core and in --cpu-procs concurrent single-thread
+processes.
 """
 from __future__ import annotations
 
 import argparse
 import json
+import os
 import sys
 import time
 from pathlib import Path
@@ -50,6 +52,43 @@ def cpu_baseline(n: int, steps: int) -> dict:
     return {"value": dof * 4 * steps / el / 1e9, "unit": "GDOF·stage/s", "cores": 1, "kind": "port",
             "sample": f"C restatement (oracle/, serial, gcc -O3) on the same Taylor-Green NS case at {n}^3 hex P3, "
                       f"{steps} RK44 steps ({dof} DOF), {el:.1f} s"}
+
+
+_CPU_CHILD = """
+import sys, time
+sys.path.insert(0, {root!r})
+from flurrypp_amd import cases
+from oracle.oracle import Oracle
+setup, U = cases.tgv3d(n={n}, order=3, viscous=1)
+o = Oracle.from_setup(setup)
+o.initializeSolution(U, 0)
+t0 = time.time()
+o.update({steps})
+print(t0, time.time(), setup.nEles * setup.nSpts)
+"""
+
+
+def cpu_baseline_processes(n: int, steps: int, procs: int) -> dict:
+    """The same sample in `procs` independent single-thread processes at once
+    (the reference's viscous path runs one thread per process): aggregate
+    DOF-stages over the span from the first timed start to the last end."""
+    import subprocess
+    env = dict(os.environ, OMP_NUM_THREADS="1")
+    code = _CPU_CHILD.format(root=str(ROOT), n=n, steps=steps)
+    ps = [subprocess.Popen([sys.executable, "-c", code], stdout=subprocess.PIPE, env=env, text=True)
+          for _ in range(procs)]
+    rows = []
+    for p in ps:
+        out, _ = p.communicate()
+        if p.returncode != 0:
+            raise RuntimeError(f"CPU baseline process failed ({p.returncode})")
+        t0, t1, dof = out.split()
+        rows.append((float(t0), float(t1), int(dof)))
+    span = max(r[1] for r in rows) - min(r[0] for r in rows)
+    work = sum(r[2] * 4 * steps for r in rows)
+    return {"value": work / span / 1e9, "unit": "GDOF·stage/s", "processes": procs, "threads_per_process": 1,
+            "sample": f"{procs} concurrent copies of the {n}^3 sample, {span:.1f} s from the first timed start "
+                      f"to the last end"}
 
 
 def batched_form(n_cases: int, seed: int, tol: float, max_steps: int, nx_hi: int, ny_hi: int) -> dict:
@@ -316,6 +355,8 @@ def main() -> int:
     ap.add_argument("--path", choices=["fused", "staged"], default="fused")
     ap.add_argument("--cpu-n", type=int, default=16)
     ap.add_argument("--cpu-steps", type=int, default=10)
+    ap.add_argument("--cpu-procs", type=int, default=min(16, os.cpu_count() or 1),
+                    help="concurrent single-thread copies for the all-cores CPU figure (<= 1 skips it)")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-mixed", action="store_true")
     ap.add_argument("--no-tune", action="store_true", help="keep the default block sizes (256)")
@@ -473,6 +514,10 @@ def main() -> int:
     if not a.no_cpu:
         log("bench: CPU baseline (oracle restatement)")
         cpu = cpu_baseline(a.cpu_n, a.cpu_steps)
+        if a.cpu_procs > 1:
+            log(f"bench: CPU baseline in {a.cpu_procs} processes")
+            cpu["processes"] = optional("cpu processes",
+                                        lambda: cpu_baseline_processes(a.cpu_n, a.cpu_steps, a.cpu_procs))
 
     out = {
         "metric": METRIC, "value": round(value, 4), "unit": "GDOF·stage/s", "n_gpus": 1, "steps": a.steps,
