@@ -30,18 +30,36 @@ class Member:
     tag: object = None
 
 
+# fhip_params fields a merged context keeps per mesh (the free stream, through
+# its boundary-state table) or sums (the element count)
+_PER_MESH = ("n_eles", "rho_bound", "u_bound", "v_bound", "w_bound", "p_bound")
+
+
+def context_key(setup) -> tuple:
+    """What one context takes from a case besides its mesh and free stream:
+    element type and order, the operators' point set and correction scheme,
+    and every other fhip_params field (dt only where it is fixed).  Cases
+    merge into one context only when these are equal."""
+    p, fp = setup.params, setup.fp
+    vals = [setup.nDims, setup.order, str(p.sptsTypeQuad).lower(), p.vcjhSchemeQuad]
+    for name, _ in nat.FhipParams._fields_:
+        if name in _PER_MESH or (name == "dt" and fp.dt_type != 0):
+            continue
+        v = getattr(fp, name)
+        vals.append(tuple(v) if isinstance(v, C.Array) else v)
+    return tuple(vals)
+
+
 class MergedSetup:
     """HostSetup-compatible union of several meshes of one order."""
 
     def __init__(self, members: list[Member]):
         s0 = members[0].setup
+        k0 = context_key(s0)
         for m in members:
-            s = m.setup
-            if (s.nDims, s.order, s.nFields) != (s0.nDims, s0.order, s0.nFields):
-                raise ValueError("a merged batch needs one element type and order")
-            for k in ("equation", "viscous", "riemannType", "timeType", "dtType", "gamma"):
-                if getattr(s.params, k) != getattr(s0.params, k):
-                    raise ValueError(f"batch members differ in {k}")
+            if context_key(m.setup) != k0:
+                raise ValueError("batch members of one context must share element type, order, operators and "
+                                 "every solver option but the free stream")
         self.params = s0.params
         self.nDims, self.nFields, self.order = s0.nDims, s0.nFields, s0.order
         self.nSpts, self.nFpts, self.nFptsFace = s0.nSpts, s0.nFpts, s0.nFptsFace
@@ -162,9 +180,7 @@ class BatchedSolver:
         # one merged context per element type, order and solver options
         groups: dict[tuple, list[int]] = {}
         for i, m in enumerate(members):
-            q = m.setup.params
-            key = (m.setup.nDims, m.setup.order, q.equation, q.viscous, q.riemannType, q.timeType, q.dtType, q.gamma)
-            groups.setdefault(key, []).append(i)
+            groups.setdefault(context_key(m.setup), []).append(i)
         self.index = []  # (group, position) of each member
         self.groups: list[BatchGroup] = []
         pos = [None] * len(members)

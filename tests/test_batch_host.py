@@ -37,3 +37,18 @@ def test_merged_setup_slices_match_members():
         f0 += n
     assert f0 == len(m.faces.eleL)
     assert m.fp.n_eles == m.nEles
+
+
+def test_members_merge_only_when_every_option_but_the_free_stream_matches():
+    import pytest
+    from flurrypp_amd.batch import context_key
+    a, Ua = cases.bump2d(nx=6, ny=3, order=2, mach=0.3, seed=1)
+    b, Ub = cases.bump2d(nx=8, ny=4, order=2, mach=0.45, seed=2)      # other mesh and free stream
+    c, Uc = cases.bump2d(nx=6, ny=3, order=2, mach=0.3, seed=1, cfl=0.9)  # other CFL
+    d, Ud = cases.bump2d(nx=6, ny=3, order=3, mach=0.3, seed=1)        # other order
+    assert context_key(a) == context_key(b)
+    assert context_key(a) != context_key(c)
+    assert context_key(a) != context_key(d)
+    MergedSetup([Member(a, Ua), Member(b, Ub)])
+    with pytest.raises(ValueError):
+        MergedSetup([Member(a, Ua), Member(c, Uc)])
