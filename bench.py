@@ -377,6 +377,8 @@ def main() -> int:
               "of the path (a mesh shards by element partition, one context per GPU).", file=sys.stderr)
         return 2
 
+    from flurrypp_amd.build import ensure_built
+    ensure_built()
     from flurrypp_amd import cases, perfmodel
     from flurrypp_amd._native import PATH_FUSED, PATH_STAGED
     from flurrypp_amd.solver import Solver

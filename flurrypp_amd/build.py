@@ -59,6 +59,15 @@ def build_all() -> None:
     build_hip()
 
 
+def ensure_built() -> None:
+    """Build whichever library is missing (a fresh checkout holds none); a
+    library that exists is used as it is."""
+    if not (LIB / "libflurry_host.so").exists():
+        build_host()
+    if not (LIB / "libflurry_hip.so").exists():
+        build_hip()
+
+
 if __name__ == "__main__":
     what = sys.argv[1:] or ["all"]
     for w in what:

@@ -8,6 +8,8 @@ if str(ROOT) not in sys.path:
 
 def pytest_configure(config):
     config.addinivalue_line("markers", "gpu: needs an MI355X (runs the HIP path)")
+    from flurrypp_amd.build import ensure_built
+    ensure_built()
 
 
 def gpu_available() -> bool:
