@@ -49,7 +49,14 @@ def cpu_baseline(n: int, steps: int) -> dict:
     o.update(steps)
     el = time.perf_counter() - t0
     dof = setup.nEles * setup.nSpts
+    model = "unknown"
+    try:
+        with open("/proc/cpuinfo") as f:
+            model = next((ln.split(":", 1)[1].strip() for ln in f if ln.startswith("model name")), model)
+    except OSError:
+        pass
     return {"value": dof * 4 * steps / el / 1e9, "unit": "GDOF·stage/s", "cores": 1, "kind": "port",
+            "host_cpu": model, "nproc": os.cpu_count(),
             "sample": f"C restatement (oracle/, serial, gcc -O3) on the same Taylor-Green NS case at {n}^3 hex P3, "
                       f"{steps} RK44 steps ({dof} DOF), {el:.1f} s"}
 
