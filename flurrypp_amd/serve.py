@@ -54,16 +54,35 @@ def build_case(r: Request):
     raise ValueError(r.kind)
 
 
-def footprint_bytes(nDims: int, order: int, nEles: int, viscous: bool = False) -> int:
-    """Device bytes one context allocates (both paths' arrays, FP64 + FP32 work)."""
+def footprint_bytes(nDims: int, order: int, nEles: int, viscous: bool = False, nRK: int = 4,
+                    pmg_source: bool = True) -> int:
+    """Device bytes one context holds once both precisions have run: the
+    staged arrays and geometry, the fused FP64 state, metric terms and face
+    tables, and the FP64 and FP32 work arrays (counted per element from
+    fhip_create, fused_setup and ensure_work), plus 25% and 16 MiB for
+    allocation granularity (measured: a 3-D P3 context of 169 MiB of arrays
+    holds 206 MiB).  The runtime's one-time cost per process (code
+    objects, scratch) is not included."""
     P1 = order + 1
-    S, N = P1 ** nDims, nDims + 2
-    F = 2 * nDims * P1 ** (nDims - 1)
-    per_ele = 8 * (N * S * (4 + 4 + 5) + N * F * 6 + nDims * N * S + nDims * nDims * S + S + nDims * F + 2 * F)
-    per_ele += 4 * (N * S + 3 * N * F) + 12 * F
+    D = nDims
+    S, N = P1 ** D, D + 2
+    F = 2 * D * P1 ** (D - 1)
+    SN, FN = S * N, F * N
+    # staged: U, U0, F_spts, divF[nRK]; U_fpts, disFn, Fn, face Fn; wave speeds, metrics, face tables
+    d = SN * (2 + D + nRK) + 4 * FN + F * (3.5 + D) + D * D * S + S + 1
+    # fused FP64: U, U0, RK accumulator, divF0; wave speeds, metrics (+1/detJ), face tables
+    d += 4 * SN + F * (2 + D) + D * D * S + 2 * S + 1.5 * F
+    # FP64 work: divF, U/Fn/disFn at the flux points
+    d += SN + 3 * FN
+    # FP32 work (half a double each): the same, stage residuals, FP32 U, FP32 geometry
+    d += 0.5 * (SN + 3 * FN + (nRK - 1) * SN + SN + D * D * S + S + D * F + F)
     if viscous:
-        per_ele += 8 * (2 * nDims * N * S + 2 * nDims * N * F) + 4 * nDims * N * F
-    return int(per_ele * nEles * 1.1) + (4 << 20)
+        d += D * SN + D * FN + FN            # staged gradients and dUc
+        d += 0.5 * F + D * F                 # fused side codes and face normals
+        d += 2 * FN + 0.5 * (2 * FN + D * F)  # dUc and LDG side parts, FP64 and FP32
+    if pmg_source:
+        d += SN                              # P-multigrid source, element-major
+    return int(8 * d * nEles * 1.25) + (16 << 20)
 
 
 @dataclass

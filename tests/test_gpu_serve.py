@@ -62,3 +62,38 @@ def test_steps_exact_when_ticks_do_not_divide_them():
     ref = s.get_U()
     s.close()
     assert np.array_equal(got[0].result, ref)
+
+
+@pytest.mark.parametrize("kind,size,order", [("vortex3d", (12,), 3), ("bump2d", (48, 16), 4)])
+def test_footprint_covers_what_a_context_allocates(kind, size, order):
+    # admission by HBM budget relies on footprint_bytes bounding a context's
+    # device memory once both precisions have run (graphs included)
+    import ctypes as C
+    from flurrypp_amd._native import PATH_FUSED, PATH_MIXED, hip_lib
+    from flurrypp_amd.serve import build_case, footprint_bytes
+    from flurrypp_amd.solver import Solver
+    hip_lib()
+    rt = C.CDLL("libamdhip64.so")
+
+    def free():
+        f, t = C.c_size_t(), C.c_size_t()
+        assert rt.hipMemGetInfo(C.byref(f), C.byref(t)) == 0
+        return f.value
+
+    def run(setup, U):
+        s = Solver(setup.params, setup=setup)
+        s.initializeSolution(U)
+        s.update(2, path=PATH_FUSED)
+        s.update(2, path=PATH_MIXED)
+        return s
+
+    r = Request(0, kind, size, order, 0.0, n_steps=2)
+    setup, U = build_case(r)
+    run(setup, U).close()   # the runtime's one-time allocations (code objects, scratch) happen here
+    f0 = free()
+    s = run(setup, U)
+    used = f0 - free()
+    s.close()
+    est = footprint_bytes(setup.nDims, setup.order, setup.nEles, bool(setup.params.viscous))
+    print(f"footprint {kind} {size} P{order}: used {used / 2**20:.1f} MiB, estimate {est / 2**20:.1f} MiB", flush=True)
+    assert used <= est, (used, est)
