@@ -755,8 +755,9 @@ int fhost_mesh_read_gmsh(const char* path, int nMap, const char* const* names, c
     }
     if (nDims != 2 && nDims != 3) throw Error("no 'fluid' physical name in the Gmsh file");
     seek("$Nodes");
-    int nV;
+    int nV = -1;
     f >> nV;
+    if (!f || nV < 0) throw Error("malformed $Nodes count");
     std::getline(f, line);
     std::vector<double> xv((size_t)nV * nDims);
     for (int i = 0; i < nV; i++) {
@@ -765,35 +766,38 @@ int fhost_mesh_read_gmsh(const char* path, int nMap, const char* const* names, c
       f >> iv >> xv[(size_t)i * nDims] >> xv[(size_t)i * nDims + 1];
       if (nDims == 3) f >> xv[(size_t)i * nDims + 2];
       else f >> z;
+      if (!f) throw Error("malformed $Nodes entry " + std::to_string(i + 1));
       std::getline(f, line);
     }
     seek("$Elements");
-    int nE;
+    int nE = -1;
     f >> nE;
+    if (!f || nE < 0) throw Error("malformed $Elements count");
     std::getline(f, line);
+    auto vertex = [&](int k) {
+      int v = 0;
+      f >> v;
+      if (!f) throw Error("malformed $Elements entry " + std::to_string(k + 1));
+      if (v < 1 || v > nV) throw Error("$Elements entry " + std::to_string(k + 1) + " names node " +
+                                       std::to_string(v) + ", outside 1.." + std::to_string(nV));
+      return v - 1;
+    };
     std::vector<int> c2v;
     std::vector<std::set<int>> bp(bcList.size());
     for (int k = 0; k < nE; k++) {
       int id, type, nTags, tag, tmp;
       f >> id >> type >> nTags >> tag;
       for (int t = 0; t < nTags - 1; t++) f >> tmp;
+      if (!f) throw Error("malformed $Elements entry " + std::to_string(k + 1));
       int b = bcIdMap.count(tag) ? bcIdMap[tag] : -2;
       if (b == -1) {
         int nn = (type == 3) ? 4 : (type == 5) ? 8 : 0;
         if (!nn) throw Error("Gmsh element type " + std::to_string(type) + " not supported (linear quads/hexes only)");
-        for (int i = 0; i < nn; i++) {
-          int v;
-          f >> v;
-          c2v.push_back(v - 1);
-        }
+        for (int i = 0; i < nn; i++) c2v.push_back(vertex(k));
       } else if (b >= 0) {
         int np = (type == 1) ? 2 : (type == 3) ? 4 : 0;
         if (!np) throw Error("Boundary Element (Face) Type Not Recognized!");
-        for (int i = 0; i < np; i++) {
-          int v;
-          f >> v;
-          bp[b].insert(v - 1);
-        }
+        for (int i = 0; i < np; i++) bp[b].insert(vertex(k));
       } else {
         throw Error("element tagged with an unknown physical id");
       }
