@@ -3,6 +3,7 @@
 // that the sources can run under AddressSanitizer and UBSan on the CPU.
 // Exit status 0 when every call succeeds.
 #include <cstdio>
+#include <string>
 #include <vector>
 
 #include "flurry_host.h"
@@ -70,8 +71,42 @@ static int mesh_case(int D, int n, const char* const* bcs, double jitter, int or
   return 0;
 }
 
-int main() {
+// A 2x1 quad mesh written as Gmsh 2.2 and read back, plus a file whose
+// element names a node that does not exist (must fail cleanly).
+static int gmsh_case(const char* dir) {
+  const std::string good = std::string(dir) + "/fhost_san_good.msh", bad = std::string(dir) + "/fhost_san_bad.msh";
+  const char* body =
+      "$MeshFormat\n2.2 0 8\n$EndMeshFormat\n$PhysicalNames\n2\n1 1 \"Wall\"\n2 2 \"FLUID\"\n"
+      "$EndPhysicalNames\n$Nodes\n6\n1 0 0 0\n2 1 0 0\n3 2 0 0\n4 0 1 0\n5 1 1 0\n6 2 1 0\n$EndNodes\n"
+      "$Elements\n8\n1 1 2 1 1 1 2\n2 1 2 1 1 2 3\n3 1 2 1 1 3 6\n4 1 2 1 1 6 5\n5 1 2 1 1 5 4\n"
+      "6 1 2 1 1 4 1\n7 3 2 2 2 1 2 5 4\n8 3 2 2 2 2 3 6 %d\n$EndElements\n";
+  const char* names[2] = {"wall", "fluid"};
+  const char* bcs[2] = {"slip_wall", "fluid"};
+  for (int pass = 0; pass < 2; pass++) {
+    const std::string& path = pass ? bad : good;
+    FILE* fp = std::fopen(path.c_str(), "w");
+    if (!fp) return 1;
+    std::fprintf(fp, body, pass ? 9 : 5);
+    std::fclose(fp);
+    fhost_mesh* m = nullptr;
+    const int rc = fhost_mesh_read_gmsh(path.c_str(), 2, names, bcs, &m);
+    std::remove(path.c_str());
+    if (pass == 1) {
+      if (rc == 0) return 1;  // node 9 of 6 must be rejected
+      continue;
+    }
+    CHK(rc);
+    CHK(fhost_mesh_process(m));
+    std::vector<double> dj((size_t)4 * 2);
+    CHK(fhost_mesh_geometry(m, 1, 0, nullptr, nullptr, nullptr, dj.data(), nullptr, nullptr));
+    fhost_mesh_destroy(m);
+  }
+  return 0;
+}
+
+int main(int argc, char** argv) {
   if (operators_all()) return 1;
+  if (gmsh_case(argc > 1 ? argv[1] : "/tmp")) return 1;
   const char* per2[4] = {"periodic", "periodic", "periodic", "periodic"};
   const char* mix2[4] = {"slip_wall", "char", "sup_out", "sup_in"};
   const char* per3[6] = {"periodic", "periodic", "periodic", "periodic", "periodic", "periodic"};
