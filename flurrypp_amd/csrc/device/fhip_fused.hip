@@ -1327,6 +1327,9 @@ struct Runner {
   // P2/P4/P5 face kernels 1.7-1.4x faster without streaming, P3 rows of 128
   // and 64 bytes faster with it).
   static constexpr bool kFaceRowsWhole = (Sh::Ff * sizeof(R)) % 64 == 0;
+  // The element kernels store rows of S or F values per field; a wave's
+  // streamed store covers whole blocks only when those rows are whole blocks.
+  static constexpr bool kEleRowsWhole = (S * sizeof(R)) % 64 == 0 && (F * sizeof(R)) % 64 == 0;
   static int& blk(Ctx& c, int k) { return fs(c).nt[sizeof(R) == 4 ? 1 : 0][k]; }
   // LDS bytes of an element kernel at block size nt (0 for the face kernels)
   static size_t lds(int k, int nt, bool visc) {
@@ -1380,7 +1383,7 @@ struct Runner {
     FCHK(hipGetLastError());
   }
   static void launch_volume(Ctx& c, int nt) {
-    const bool sm = fs(c).stream;
+    const bool sm = fs(c).stream && kEleRowsWhole;
     if (c.prm.viscous) sm ? launch_volume_t<true, true>(c, nt) : launch_volume_t<true, false>(c, nt);
     else sm ? launch_volume_t<false, true>(c, nt) : launch_volume_t<false, false>(c, nt);
   }
@@ -1398,7 +1401,7 @@ struct Runner {
     RkB bs{};
     for (int j = 0; j < c.z.nRK && j < 8; j++) bs.b[j] = c.prm.rk_b[j];
     R* rst = (w.Rst && !st.use_src) ? w.Rst : nullptr;
-    auto* kern = st.stream ? k_rk<D, P, R, true> : k_rk<D, P, R, false>;
+    auto* kern = (st.stream && kEleRowsWhole) ? k_rk<D, P, R, true> : k_rk<D, P, R, false>;
     hipLaunchKernelGGL(kern, dim3(nblk(c.z.E, nt / S)), dim3(nt), lds(kRk, nt, false), c.stream, c.z.E, mode, a, b,
                        w.divFe, w.Fne, w.disFne, st.detJe, c.d_dt, c.eleDt,
                        c.prm.dt_type == 2 ? 1 : (c.prm.dt_type == 1 && st.nMesh > 1 ? 2 : 0), st.Ue, st.U0e, st.Uacc,
