@@ -255,6 +255,8 @@ __global__ __launch_bounds__(kMaxBlock) void k_extrap(int E, const double* __res
 // whose arrays are far larger than the caches (NT) they stream past them
 // (nontemporal); on smaller ones they stay plain, since there the next kernel
 // of the stage finds them in L2 or the infinity cache (measured both ways).
+// A kernel streams only where the rows it stores are whole 64-byte blocks
+// (Runner::kFaceRowsWhole, kEleRowsWhole).
 template <bool NT, class T>
 __device__ __forceinline__ T ld_s(const T* p) {
   if constexpr (NT) return __builtin_nontemporal_load(p);
