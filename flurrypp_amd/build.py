@@ -8,6 +8,7 @@ Run ``python -m flurrypp_amd.build`` to rebuild.
 """
 from __future__ import annotations
 
+import hashlib
 import os
 import subprocess
 import sys
@@ -35,11 +36,36 @@ def _run(cmd: list[str], cwd: Path | None = None) -> None:
     subprocess.run(cmd, check=True, cwd=cwd)
 
 
+HOST_FLAGS = ["-O2", "-std=c++17", "-fPIC", "-shared", "-fopenmp"]
+
+
+def _inputs(kind: str) -> list[Path]:
+    """Every file a library is built from: its sources, the headers they may
+    include and this recipe."""
+    sub = CSRC / ("device" if kind == "hip" else "host")
+    files = sorted(p for p in sub.iterdir() if p.suffix in (".hip", ".cpp", ".hpp", ".h"))
+    return files + sorted(INC.glob("*.h")) + [Path(__file__)]
+
+
+def source_hash(kind: str) -> str:
+    h = hashlib.sha256()
+    for p in _inputs(kind):
+        h.update(p.name.encode())
+        h.update(p.read_bytes())
+    return h.hexdigest()
+
+
+def _stamp(out: Path) -> Path:
+    return out.with_name(out.name + ".srchash")
+
+
 def build_host() -> Path:
     LIB.mkdir(exist_ok=True)
     out = LIB / "libflurry_host.so"
     srcs = [str(CSRC / "host" / "fr_basis.cpp"), str(CSRC / "host" / "fr_mesh.cpp")]
-    _run(["g++", "-O2", "-std=c++17", "-fPIC", "-shared", "-fopenmp", f"-I{INC}", *srcs, "-o", str(out)])
+    digest = source_hash("host")
+    _run(["g++", *HOST_FLAGS, f"-I{INC}", *srcs, "-o", str(out)])
+    _stamp(out).write_text(digest + "\n")
     return out
 
 
@@ -49,9 +75,19 @@ def build_hip() -> Path:
     dev = CSRC / "device"
     srcs = [str(dev / f) for f in ("fhip_api.hip", "fhip_staged.hip", "fhip_fused.hip")]
     tmp = out.with_suffix(".so.tmp")
+    digest = source_hash("hip")
     _run([HIPCC, *HIP_FLAGS, f"-I{INC}", f"-I{dev}", *srcs, "-o", str(tmp)])
     os.replace(tmp, out)
+    _stamp(out).write_text(digest + "\n")
     return out
+
+
+def is_current(kind: str) -> bool:
+    """True when the library exists and was built from the sources as they
+    are now (the hash stored beside it at build time matches)."""
+    out = LIB / ("libflurry_hip.so" if kind == "hip" else "libflurry_host.so")
+    st = _stamp(out)
+    return out.exists() and st.exists() and st.read_text().strip() == source_hash(kind)
 
 
 def build_all() -> None:
@@ -60,11 +96,11 @@ def build_all() -> None:
 
 
 def ensure_built() -> None:
-    """Build whichever library is missing (a fresh checkout holds none); a
-    library that exists is used as it is."""
-    if not (LIB / "libflurry_host.so").exists():
+    """Build whichever library is missing or older than its sources (a fresh
+    checkout holds none), so that no run uses a stale library."""
+    if not is_current("host"):
         build_host()
-    if not (LIB / "libflurry_hip.so").exists():
+    if not is_current("hip"):
         build_hip()
 
 

@@ -527,6 +527,15 @@ static double ldg_pen(const or_params* p, const double* n) { /* src/face.cpp:248
   return pf;
 }
 
+/* face::ldgSolution, src/face.cpp:588-615: the common solution UC at one
+ * face flux point (bnd: plain average with the boundary state) */
+static void ldg_common_u(const or_params* p, int bnd, const double* UL, const double* UR, const double* n,
+                         double* UC) {
+  double pf = bnd ? 0. : ldg_pen(p, n);
+  for (int k = 0; k < p->n_fields; k++)
+    UC[k] = bnd ? 0.5 * (UL[k] + UR[k]) : 0.5 * (UL[k] + UR[k]) - pf * (UL[k] - UR[k]);
+}
+
 /* solver::calcInviscidFlux_faces -> face::calcInviscidFlux, src/face.cpp:163-208 */
 static void calc_inviscid_flux_faces(or_ctx* c) {
   int D = c->D, N = c->N, E = c->E, F = c->F, P1f = c->P1f;
@@ -553,11 +562,8 @@ static void calc_inviscid_flux_faces(or_ctx* c) {
       }
       if (p->viscous) {
         double UC[5];
-        double pf = bnd ? 0. : ldg_pen(p, n);
-        for (int k = 0; k < N; k++) {
-          UC[k] = bnd ? 0.5 * (UL[k] + UR[k]) : 0.5 * (UL[k] + UR[k]) - pf * (UL[k] - UR[k]);
-          c->dUc[I3(fl, eL, k, E, N)] = UC[k] - UL[k];
-        }
+        ldg_common_u(p, bnd, UL, UR, n, UC);
+        for (int k = 0; k < N; k++) c->dUc[I3(fl, eL, k, E, N)] = UC[k] - UL[k];
         if (!bnd)
           for (int k = 0; k < N; k++) c->dUc[I3(fr, eR, k, E, N)] = UC[k] - UR[k];
       }
@@ -616,8 +622,78 @@ static void calc_viscous_flux_spts(or_ctx* c) { /* :777-811 */
     }
 }
 
-/* face::calcViscousFlux, src/face.cpp:210-311, with boundFace::applyViscousBCs
- * (src/boundFace.cpp:316-352) */
+/* boundFace::applyViscousBCs, src/boundFace.cpp:316-352: the adiabatic wall
+ * copies the interior gradient and rewrites its energy column so that the
+ * temperature gradient vanishes */
+static void adiabatic_grad(const or_params* p, const double* UL, const double* UR, double gL[3][5],
+                           double gR[3][5]) {
+  int D = p->n_dims;
+  for (int d = 0; d < D; d++)
+    for (int k = 0; k < p->n_fields; k++) gR[d][k] = gL[d][k];
+  double rhovSq = 0.;
+  for (int d = 0; d < D; d++) rhovSq += UL[d + 1] * UL[d + 1];
+  double pL = (p->gamma - 1.) * (UL[D + 1] - 0.5 * rhovSq / UL[0]);
+  double pR = pL;
+  double e = pR / ((p->gamma - 1.) * UR[0]);
+  double gv[3][3];
+  for (int d1 = 0; d1 < D; d1++)
+    for (int d2 = 0; d2 < D; d2++) gv[d1][d2] = (gR[d1][d2 + 1] - gR[d1][0] * UR[d2 + 1] / UR[0]) / UR[0];
+  double vSq = 0.;
+  for (int d = 0; d < D; d++) vSq += UR[d + 1] * UR[d + 1] / (UR[0] * UR[0]);
+  for (int d1 = 0; d1 < D; d1++) {
+    gR[d1][D + 1] = (e + 0.5 * vSq) * gR[d1][0];
+    for (int d2 = 0; d2 < D; d2++) gR[d1][D + 1] += UR[d2 + 1] * gv[d2][d1];
+  }
+}
+
+/* face::calcViscousFlux, src/face.cpp:216-293, at one face flux point: the
+ * LDG common viscous flux Fc dotted with the normal, added to Fn[N].  bc < 0
+ * for interior points (UR, gR the right side's values); on boundaries UR is
+ * the boundary state and gR is set here (boundFace::getRightGradient,
+ * applyViscousBCs, src/boundFace.cpp:56-63,316-352). */
+static void ldg_point(const or_params* p, int bc, const double* UL, const double* UR, double gL[3][5],
+                      double gR[3][5], const double* n, double* Fn) {
+  int D = p->n_dims, N = p->n_fields, bnd = (bc >= 0);
+  double FL[3][5], FR[3][5], Fc[3][5];
+  if (bnd) {
+    if (bc == 10) adiabatic_grad(p, UL, UR, gL, gR);
+    else
+      for (int d = 0; d < D; d++)
+        for (int k = 0; k < N; k++) gR[d][k] = gL[d][k];
+    if (bc == 10) {
+      viscous_flux(UR, gR, FR, p);
+      for (int d = 0; d < D; d++)
+        for (int k = 0; k < N; k++) Fc[d][k] = FR[d][k] + p->tau * n[d] * (UL[k] - UR[k]);
+    } else {
+      viscous_flux(UL, gL, FL, p);
+      for (int d = 0; d < D; d++)
+        for (int k = 0; k < N; k++) Fc[d][k] = FL[d][k] + p->tau * n[d] * (UL[k] - UR[k]);
+    }
+  } else {
+    viscous_flux(UL, gL, FL, p);
+    viscous_flux(UR, gR, FR, p);
+    double pf = ldg_pen(p, n);
+    double nx = n[0], ny = n[1], nz = (D == 3) ? n[2] : 0;
+    for (int k = 0; k < N; k++) {
+      double dF0 = FL[0][k] - FR[0][k], dF1 = FL[1][k] - FR[1][k];
+      double dU = UL[k] - UR[k];
+      if (D == 2) {
+        Fc[0][k] = 0.5 * (FL[0][k] + FR[0][k]) + pf * nx * (nx * (dF0) + ny * (dF1)) + p->tau * nx * (dU);
+        Fc[1][k] = 0.5 * (FL[1][k] + FR[1][k]) + pf * ny * (nx * (dF0) + ny * (dF1)) + p->tau * ny * (dU);
+      } else {
+        double dF2 = FL[1][k] - FR[2][k]; /* src/face.cpp:277 */
+        double s = nx * (dF0) + ny * (dF1) + nz * (dF2);
+        Fc[0][k] = 0.5 * (FL[0][k] + FR[0][k]) + pf * nx * s + p->tau * nx * (dU);
+        Fc[1][k] = 0.5 * (FL[1][k] + FR[1][k]) + pf * ny * s + p->tau * ny * (dU);
+        Fc[2][k] = 0.5 * (FL[2][k] + FR[2][k]) + pf * nz * s + p->tau * nz * (dU);
+      }
+    }
+  }
+  for (int d = 0; d < D; d++)
+    for (int k = 0; k < N; k++) Fn[k] += Fc[d][k] * n[d];
+}
+
+/* solver::calcViscousFlux_faces -> face::calcViscousFlux for every face */
 static void calc_viscous_flux_faces(or_ctx* c) {
   int D = c->D, N = c->N, E = c->E, P1f = c->P1f;
   const or_params* p = &c->p;
@@ -626,7 +702,7 @@ static void calc_viscous_flux_faces(or_ctx* c) {
     int eL = c->fEL[f], eR = c->fER[f], bnd = (eR < 0), bc = c->fBC[f];
     for (int i = 0; i < P1f; i++) {
       int fl = c->fLL[f] * P1f + i, fr = -1;
-      double UL[5], UR[5], n[3], gL[3][5], gR[3][5], FL[3][5], FR[3][5], Fc[3][5];
+      double UL[5], UR[5], n[3], gL[3][5], gR[3][5];
       double* Fn = &c->faceFn[((size_t)f * P1f + i) * N];
       for (int k = 0; k < N; k++) UL[k] = c->Uf[I3(fl, eL, k, E, N)];
       for (int d = 0; d < D; d++) n[d] = c->norm[I3(fl, eL, d, E, D)];
@@ -634,62 +710,13 @@ static void calc_viscous_flux_faces(or_ctx* c) {
         for (int k = 0; k < N; k++) gL[d][k] = c->dUf[d * nF + I3(fl, eL, k, E, N)];
       if (bnd) {
         apply_bcs(p, bc, UL, n, UR);
-        if (bc == 10) { /* adiabatic wall: copy gradients, zero temperature gradient */
-          for (int d = 0; d < D; d++)
-            for (int k = 0; k < N; k++) gR[d][k] = gL[d][k];
-          double rhovSq = 0.;
-          for (int d = 0; d < D; d++) rhovSq += UL[d + 1] * UL[d + 1];
-          double pL = (p->gamma - 1.) * (UL[D + 1] - 0.5 * rhovSq / UL[0]);
-          double pR = pL;
-          double e = pR / ((p->gamma - 1.) * UR[0]);
-          double gv[3][3];
-          for (int d1 = 0; d1 < D; d1++)
-            for (int d2 = 0; d2 < D; d2++) gv[d1][d2] = (gR[d1][d2 + 1] - gR[d1][0] * UR[d2 + 1] / UR[0]) / UR[0];
-          double vSq = 0.;
-          for (int d = 0; d < D; d++) vSq += UR[d + 1] * UR[d + 1] / (UR[0] * UR[0]);
-          for (int d1 = 0; d1 < D; d1++) {
-            gR[d1][D + 1] = (e + 0.5 * vSq) * gR[d1][0];
-            for (int d2 = 0; d2 < D; d2++) gR[d1][D + 1] += UR[d2 + 1] * gv[d2][d1];
-          }
-        }
       } else {
         fr = c->fptR[(size_t)f * P1f + i];
         for (int k = 0; k < N; k++) UR[k] = c->Uf[I3(fr, eR, k, E, N)];
         for (int d = 0; d < D; d++)
           for (int k = 0; k < N; k++) gR[d][k] = c->dUf[d * nF + I3(fr, eR, k, E, N)];
       }
-      if (bnd) {
-        if (bc == 10) {
-          viscous_flux(UR, gR, FR, p);
-          for (int d = 0; d < D; d++)
-            for (int k = 0; k < N; k++) Fc[d][k] = FR[d][k] + p->tau * n[d] * (UL[k] - UR[k]);
-        } else {
-          viscous_flux(UL, gL, FL, p);
-          for (int d = 0; d < D; d++)
-            for (int k = 0; k < N; k++) Fc[d][k] = FL[d][k] + p->tau * n[d] * (UL[k] - UR[k]);
-        }
-      } else {
-        viscous_flux(UL, gL, FL, p);
-        viscous_flux(UR, gR, FR, p);
-        double pf = ldg_pen(p, n);
-        double nx = n[0], ny = n[1], nz = (D == 3) ? n[2] : 0;
-        for (int k = 0; k < N; k++) {
-          double dF0 = FL[0][k] - FR[0][k], dF1 = FL[1][k] - FR[1][k];
-          double dU = UL[k] - UR[k];
-          if (D == 2) {
-            Fc[0][k] = 0.5 * (FL[0][k] + FR[0][k]) + pf * nx * (nx * (dF0) + ny * (dF1)) + p->tau * nx * (dU);
-            Fc[1][k] = 0.5 * (FL[1][k] + FR[1][k]) + pf * ny * (nx * (dF0) + ny * (dF1)) + p->tau * ny * (dU);
-          } else {
-            double dF2 = FL[1][k] - FR[2][k]; /* src/face.cpp:277 */
-            double s = nx * (dF0) + ny * (dF1) + nz * (dF2);
-            Fc[0][k] = 0.5 * (FL[0][k] + FR[0][k]) + pf * nx * s + p->tau * nx * (dU);
-            Fc[1][k] = 0.5 * (FL[1][k] + FR[1][k]) + pf * ny * s + p->tau * ny * (dU);
-            Fc[2][k] = 0.5 * (FL[2][k] + FR[2][k]) + pf * nz * s + p->tau * nz * (dU);
-          }
-        }
-      }
-      for (int d = 0; d < D; d++)
-        for (int k = 0; k < N; k++) Fn[k] += Fc[d][k] * n[d];
+      ldg_point(p, bnd ? bc : -1, UL, UR, gL, gR, n, Fn);
       double dAL = c->da[(size_t)fl * E + eL];
       for (int k = 0; k < N; k++) c->Fn[I3(fl, eL, k, E, N)] = Fn[k] * dAL;
       if (!bnd) {
@@ -831,6 +858,7 @@ int or_set_dt(or_ctx* c, double dt) {
 }
 
 int or_get_iter(or_ctx* c) { return c->iter; }
+double or_get_dt(or_ctx* c) { return c->dt; }
 
 /* flurry main loop body (src/flurry.cpp:135-138) around solver::update
  * (src/solver.cpp:319-363), static grid */
@@ -986,3 +1014,20 @@ void or_hook_roe2d(const or_params* p, const double* UL, const double* UR, const
 void or_hook_bc(const or_params* p, int bc, const double* UL, const double* n, double* UR) { apply_bcs(p, bc, UL, n, UR); }
 
 double or_hook_ldg_pen(const or_params* p, const double* n) { return ldg_pen(p, n); }
+
+void or_hook_ldg_solution(const or_params* p, int bnd, const double* UL, const double* UR, const double* n,
+                          double* UC) {
+  ldg_common_u(p, bnd, UL, UR, n, UC);
+}
+
+void or_hook_ldg_flux(const or_params* p, int bc, const double* UL, const double* UR, const double* gL,
+                      const double* gR, const double* n, double* Fn, double* gR_used) {
+  double gl[3][5], gr[3][5], ur[5];
+  memcpy(gl, gL, sizeof gl);
+  memcpy(gr, gR, sizeof gr);
+  if (bc >= 0) apply_bcs(p, bc, UL, n, ur);
+  else memcpy(ur, UR, sizeof ur);
+  for (int k = 0; k < 5; k++) Fn[k] = 0.;
+  ldg_point(p, bc, UL, ur, gl, gr, n, Fn);
+  if (gR_used) memcpy(gR_used, gr, sizeof gr);
+}

@@ -70,6 +70,7 @@ int or_update(or_ctx* c, int n_steps, const double* src);
 int or_pmg_cycle(or_ctx** lv, int nl, const double* const* R, const double* const* Pro, int smooth);
 int or_residual_norm(or_ctx* c, int res_type, double* out);
 int or_get_iter(or_ctx* c);
+double or_get_dt(or_ctx* c);  /* the time step of the last update / calcDt */
 
 /* Pointwise hooks used by the oracle's own tests (src/flux.cpp:37-210,
  * src/face.cpp:314-495,588-615, src/boundFace.cpp:65-314).  F arrays are
@@ -80,6 +81,15 @@ double or_hook_rusanov(const or_params* p, const double* UL, const double* UR, c
 void or_hook_roe2d(const or_params* p, const double* UL, const double* UR, const double* n, double* Fn);
 void or_hook_bc(const or_params* p, int bc, const double* UL, const double* n, double* UR);
 double or_hook_ldg_pen(const or_params* p, const double* n);
+/* ldgSolution at one point (src/face.cpp:588-615) */
+void or_hook_ldg_solution(const or_params* p, int bnd, const double* UL, const double* UR, const double* n,
+                          double* UC);
+/* LDG common viscous flux . n at one point (src/face.cpp:216-293); bc < 0:
+ * interior point with right state UR, gR; bc >= 0: boundary point, the right
+ * state comes from the boundary condition and gR is ignored.  gR_used
+ * ([3][5], may be NULL) returns the right gradient the flux used. */
+void or_hook_ldg_flux(const or_params* p, int bc, const double* UL, const double* UR, const double* gL,
+                      const double* gR, const double* n, double* Fn, double* gR_used);
 
 #ifdef __cplusplus
 }

@@ -72,6 +72,8 @@ def lib() -> C.CDLL:
         L.or_set_dt.argtypes = [C.c_void_p, C.c_double]
         L.or_update.argtypes = [C.c_void_p, C.c_int, _dp]
         L.or_residual_norm.argtypes = [C.c_void_p, C.c_int, _dp]
+        L.or_get_dt.argtypes = [C.c_void_p]
+        L.or_get_dt.restype = C.c_double
         L.or_pmg_cycle.argtypes = [C.POINTER(C.c_void_p), C.c_int, C.POINTER(_dp), C.POINTER(_dp), C.c_int]
         _lib = L
     return _lib
@@ -180,6 +182,11 @@ class Oracle:
 
     def set_dt(self, dt):
         _chk(lib().or_set_dt(self._h, dt))
+
+    @property
+    def dt(self) -> float:
+        """The time step of the last update (or calcDt)."""
+        return lib().or_get_dt(self._h)
 
     def update(self, n=1, src=None):
         _chk(lib().or_update(self._h, n, None if src is None else _d(src).ctypes.data_as(_dp)))

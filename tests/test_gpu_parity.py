@@ -186,8 +186,12 @@ def test_trajectory(case, path):
     err = rel_l2(s.get_U(), o.get_U())
     print(f"{name} path={path} U rel_l2 after {nsteps} steps = {err:.3e}", flush=True)
     assert err < TOL_TRAJ
-    if p.dtType != 0:
-        assert abs(s.dt - o.calcDt(False)) <= 1e-12 * abs(s.dt) or True
+    if p.dtType == 1:
+        # global CFL dt of the last step (src/solver.cpp:447-468)
+        assert abs(s.dt - o.dt) <= 1e-12 * abs(o.dt), (s.dt, o.dt)
+    elif p.dtType == 2:
+        e_s, e_o = s.get("ele_dt"), o.get("ele_dt")
+        assert rel_l2(e_s, e_o) <= 1e-12
     r_s, r_o = s.residual_norm(2), o.residual_norm(2)
     assert np.allclose(r_s, r_o, rtol=1e-10, atol=0)
 
