@@ -781,6 +781,19 @@ int fhip_residual_norm(fhip_ctx* h, int res_type, double* out) {
   });
 }
 
+int fhip_residual_norm_noalloc(fhip_ctx* h, int res_type, double* out) {
+  return guard([&] {
+    require(h && out, "null argument");
+    require(res_type >= 1 && res_type <= 3, "resType must be 1, 2 or 3");
+    try {
+      fhip::residual_norm_noalloc(h->c, res_type, out);
+    } catch (const std::length_error& e) {
+      throw ApiError(FHIP_ESIZE, e.what());
+    }
+    return 0;
+  });
+}
+
 int fhip_check_state(fhip_ctx* h) {
   return guard([&] {
     require(h, "null context");

@@ -148,6 +148,12 @@ void op_apply(Ctx& c, const char* name, int m, int k, const double* A, const dou
 void sub_array(Ctx& c, double* out, const double* a, const double* b, long long n);  // out = a; out -= b
 void add_array(Ctx& c, double* y, const double* x, long long n);                     // y += x
 void residual_norm(Ctx& c, int res_type, double* out_host);
+// The same without allocating: partial sums in a static device array of
+// kNoAllocScratch doubles per GPU (std::length_error beyond it).
+constexpr long long kNoAllocScratch = 16384;  // 128 KiB per GPU
+constexpr int kNoAllocMaxDevices = 64;
+void residual_norm_noalloc(Ctx& c, int res_type, double* out_host);
+double* noalloc_scratch(int device);  // device address of that GPU's scratch array
 int check_state(Ctx& c);
 
 // fused path (fhip_fused.hip)

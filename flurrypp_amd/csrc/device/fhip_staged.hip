@@ -8,7 +8,9 @@
 
 #include <algorithm>
 #include <cstdio>
+#include <mutex>
 #include <stdexcept>
+#include <string>
 
 namespace fhip {
 
@@ -548,6 +550,66 @@ void residual_norm(Ctx& c, int type, double* out) {
     for (unsigned b = 0; b < nb; b++) r = (type == 3) ? std::max(r, h[(size_t)b * z.N + k]) : r + h[(size_t)b * z.N + k];
     out[k] = (type == 2) ? std::sqrt(r) : r;
   }
+}
+
+// ---------------------------------------------------------------------------
+// Residual norm without allocation, for callers that cannot call hipMalloc.
+// The per-block partial sums and the final N values live in `scratch`, a
+// file-scope device array of kNoAllocScratch doubles that every GPU the
+// library runs on holds until the process ends.  Host code never passes the
+// array's name to a kernel (on the host it is a stand-in, not a device
+// address): the device address of each GPU's copy is looked up once, with
+// that GPU current, and kept in g_scratch_ptr.  Calls on one GPU share its
+// copy, so calls that overlap on one GPU race; nothing in it has to start at
+// zero (every value read was written earlier in the same call).
+static __device__ __align__(256) double scratch[kNoAllocScratch];
+static std::mutex g_scratch_mu;
+static double* g_scratch_ptr[kNoAllocMaxDevices] = {};
+
+double* noalloc_scratch(int device) {
+  if (device < 0 || device >= kNoAllocMaxDevices) throw std::invalid_argument("device ordinal out of range");
+  std::lock_guard<std::mutex> g(g_scratch_mu);
+  if (!g_scratch_ptr[device]) {
+    int cur = -1;
+    HIPCHK(hipGetDevice(&cur));
+    if (cur != device) HIPCHK(hipSetDevice(device));
+    void* p = nullptr;
+    const hipError_t e = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+    if (cur != device) (void)hipSetDevice(cur);
+    HIPCHK(e);
+    g_scratch_ptr[device] = static_cast<double*>(p);
+  }
+  return g_scratch_ptr[device];
+}
+
+// The host sum of residual_norm done on the device: one thread per field
+// walks the block partials in ascending block order, as the host loop does,
+// so both give the same bits.
+__global__ void k_res_final(int nb, int N, int type, const double* __restrict__ part, double* __restrict__ out) {
+  const int k = threadIdx.x;
+  if (k >= N) return;
+  double r = 0;
+  for (int b = 0; b < nb; b++) r = (type == 3) ? fmax(r, part[(long long)b * N + k]) : r + part[(long long)b * N + k];
+  out[k] = (type == 2) ? sqrt(r) : r;
+}
+
+void residual_norm_noalloc(Ctx& c, int type, double* out) {
+  const Sizes& z = c.z;
+  const unsigned nb = blocks_for(z.E, 256);
+  const long long need = ((long long)nb + 1) * z.N;
+  if (need > kNoAllocScratch)
+    throw std::length_error("fhip_residual_norm_noalloc: " + std::to_string(z.E) + " elements need " +
+                            std::to_string(need) + " doubles of scratch, more than its maximum of " +
+                            std::to_string(kNoAllocScratch) + " (at most " +
+                            std::to_string((kNoAllocScratch / z.N - 1) * 256) + " elements)");
+  double* part = noalloc_scratch(c.device);
+  double* fin = part + (long long)nb * z.N;
+  hipLaunchKernelGGL(k_res_norm, dim3(nb), dim3(256), 0, c.stream, z.E, z.S, z.N, type, c.divF, c.detj, c.ws, part);
+  HIPCHK(hipGetLastError());
+  hipLaunchKernelGGL(k_res_final, dim3(1), dim3(64), 0, c.stream, (int)nb, z.N, type, part, fin);
+  HIPCHK(hipGetLastError());
+  HIPCHK(hipMemcpyAsync(out, fin, (size_t)z.N * sizeof(double), hipMemcpyDeviceToHost, c.stream));
+  HIPCHK(hipStreamSynchronize(c.stream));
 }
 
 int check_state(Ctx& c) {

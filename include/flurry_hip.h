@@ -205,6 +205,16 @@ int fhip_get_block_sizes(fhip_ctx* ctx, int path, int* block_sizes);
 /* Residual norm over divF_spts[0] (resType 1, 2 or 3): out[nFields]. */
 int fhip_residual_norm(fhip_ctx* ctx, int res_type, double* out);
 
+/* fhip_residual_norm without any allocation.  No counterpart in the
+ * reference: it is for callers that cannot allocate device memory (tight
+ * serving loops, harnesses that forbid hipMalloc).  Its partial sums live in
+ * a static device array of 16384 doubles (128 KiB) that each GPU the library
+ * runs on holds until the process ends: contexts of up to
+ * (16384 / nFields - 1) * 256 elements (838,400 in 3-D, 1,048,320 in 2-D);
+ * larger ones return FHIP_ESIZE.  Same result as fhip_residual_norm, bit for
+ * bit.  Calls that overlap on one GPU share that array and race. */
+int fhip_residual_norm_noalloc(fhip_ctx* ctx, int res_type, double* out);
+
 /* Batched contexts: residual norm of every mesh, out[n_meshes][nFields],
  * from divF_spts[0] of the last fused step; and per-mesh activity flags
  * (0 freezes a mesh: its solution is left unchanged by fhip_update). */
