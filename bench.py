@@ -9,9 +9,11 @@ One benchmark step is one RK44 time step (4 stages).  The metric counts
 DOF x stages per second (DOF = elements x solution points).
 
 Prints one JSON line.  The timed region runs `--steps` RK steps through
-fhip_update on the device, with every kernel bracketed by HIP events on the
-context's stream (no synchronisation inside the region); the per-kernel
-averages give the roofline fraction of the dominant kernel.  The CPU
+fhip_update on the device in its production mode (one captured HIP graph
+replayed per RK step, no synchronisation inside the region).  A further
+`--kernel-steps` pass launches kernel by kernel with HIP events on the
+context's stream around every launch; its per-kernel averages give the
+roofline fraction of the dominant kernel.  The CPU
 baseline times the C restatement in oracle/ (serial) on a smaller Taylor-Green
 mesh on this host, on one core and in --cpu-procs concurrent single-thread
 processes.
@@ -98,13 +100,15 @@ def cpu_baseline_processes(n: int, steps: int, procs: int) -> dict:
                       f"to the last end"}
 
 
-def batched_form(n_cases: int, seed: int, tol: float, max_steps: int, nx_hi: int, ny_hi: int) -> dict:
+def batched_form(n_cases: int, seed: int, tol: float, max_steps: int, nx_hi: int, ny_hi: int,
+                 cfl: list[float] | None = None) -> dict:
     """Independent 2-D Euler Gaussian-bump channels (sizes, orders 1..5 and
     inlet Mach 0.2-0.5 drawn seeded), merged per order, with the settings of
     FlurryPP's own P-multigrid channel case (tests/euler/channel/input_channelPMG:
     tuned steady RK (timeType 6), local CFL 0.9, lowOrder 0, smoothSteps 2),
     run until each case's L2 density residual is <= tol; converged and
-    diverged cases are frozen, the rest stop at max_steps."""
+    diverged cases are frozen, the rest stop at max_steps.  cfl[P-1] is the
+    local CFL number of order P (default 0.9 for every order)."""
     from flurrypp_amd import cases
     from flurrypp_amd.batch import BatchedSolver, Member
     rng = np.random.default_rng(seed)
@@ -112,7 +116,8 @@ def batched_form(n_cases: int, seed: int, tol: float, max_steps: int, nx_hi: int
     for i in range(n_cases):
         nx, ny = int(rng.integers(16, nx_hi + 1)), int(rng.integers(8, ny_hi + 1))
         P, M = int(rng.integers(1, 6)), float(rng.uniform(0.2, 0.5))
-        s, U = cases.bump2d(nx=nx, ny=ny, order=P, mach=M, seed=seed * 100003 + i, time_type=6, cfl=0.9)
+        s, U = cases.bump2d(nx=nx, ny=ny, order=P, mach=M, seed=seed * 100003 + i, time_type=6,
+                            cfl=cfl[P - 1] if cfl else 0.9)
         ms.append(Member(s, U, (nx, ny, P, M)))
     t = time.perf_counter()
     b = BatchedSolver(ms, pmg=True, low_order=0, smooth_steps=2, autotune=True)
@@ -130,12 +135,19 @@ def batched_form(n_cases: int, seed: int, tol: float, max_steps: int, nx_hi: int
                 for P in range(1, 6)}
     tc = out["time_s"][conv]
     dof = sum(m.setup.nEles * m.setup.nSpts for m in ms)
+    cfls = ", ".join(f"P{P} {cfl[P - 1] if cfl else 0.9:g}" for P in range(1, 6))
     return {
         "workload": f"{n_cases} 2-D Euler Gaussian-bump channels, quads 16..{nx_hi} x 8..{ny_hi}, P 1..5, "
-                    f"Mach 0.2-0.5, timeType 6, local CFL 0.9, P-multigrid (orders 0..P-1, 2 smoothing steps), "
+                    f"Mach 0.2-0.5, timeType 6, local CFL ({cfls}), P-multigrid (orders 0..P-1, 2 smoothing steps), "
                     f"steady to L2 residual {tol:g} within {max_steps} steps",
         "cases": n_cases, "dof": dof, "converged": int(conv.sum()), "diverged": int(div.sum()),
+        "unconverged": int(n_cases - conv.sum() - div.sum()),
+        "fractions": {"converged": round(float(conv.mean()), 4), "diverged": round(float(div.mean()), 4),
+                      "unconverged": round(float(1 - conv.mean() - div.mean()), 4)},
         "by_order": by_order, "wall_s": round(out["wall_s"], 3),
+        # time to converged over the converged cases only; the others are
+        # counted above (diverged cases are frozen when their residual turns
+        # NaN, unconverged ones run to max_steps)
         "p50_s": float(np.percentile(tc, 50)) if len(tc) else None,
         "p99_s": float(np.percentile(tc, 99)) if len(tc) else None,
         "steps_p50": int(np.percentile(out["steps"], 50)), "steps_max": int(out["steps"].max()),
@@ -144,18 +156,22 @@ def batched_form(n_cases: int, seed: int, tol: float, max_steps: int, nx_hi: int
     }
 
 
-def served_form(n_req: int, rate: float, seed: int) -> dict:
-    """Poisson arrivals: 70% steady 2-D bump cases (P-multigrid), 30% unsteady
-    3-D periodic Euler vortex columns; half FP64, half mixed precision."""
+def served_form(n_req: int, rates: list[float], seed: int) -> dict:
+    """Poisson arrivals of the same request mix at each rate in `rates`: 70%
+    steady 2-D bump channels (16..128 x 8..48 quads, P 1..5, Mach 0.2-0.5,
+    P-multigrid, residual 1e-8 within 2000 steps), 30% unsteady 3-D periodic
+    Euler vortex columns (8..40^3 hexes, P 2..4, 100 steps); half FP64, half
+    mixed precision (mixed steady requests finish in FP64)."""
     from flurrypp_amd.serve import Server, poisson_mix
-    reqs = poisson_mix(n_req, rate=rate, seed=seed, steady_tol=1e-8, unsteady_steps=100, n3d=(8, 24), p3d=(2, 4),
-                       nx2d=(16, 64), ny2d=(8, 24), p2d=(1, 4), max_steps=2000)
-    srv = Server(hbm_budget=200e9, max_active=32, steps_per_tick=10, check_every=50, pmg=True, log=log)
-    st = srv.run(reqs)
-    st["workload"] = (f"{n_req} requests, Poisson rate {rate:g}/s: 70% steady 2-D bump (16..64 x 8..24, P 1..4, "
-                      f"residual 1e-8 within 2000 steps, P-multigrid), 30% 3-D Euler vortex (8..24^3, P 2..4, "
-                      f"100 steps); FP64/mixed 50/50")
-    return st
+    out = {"workload": f"{n_req} requests per rate: 70% steady 2-D bump (16..128 x 8..48, P 1..5, residual 1e-8 "
+                       f"within 2000 steps, P-multigrid), 30% 3-D Euler vortex (8..40^3, P 2..4, 100 steps); "
+                       f"FP64/mixed 50/50", "rates": {}}
+    for rate in rates:
+        reqs = poisson_mix(n_req, rate=rate, seed=seed, steady_tol=1e-8, unsteady_steps=100, n3d=(8, 40),
+                           p3d=(2, 4), nx2d=(16, 128), ny2d=(8, 48), p2d=(1, 5), max_steps=2000)
+        srv = Server(hbm_budget=200e9, max_active=32, steps_per_tick=10, check_every=50, pmg=True, log=log)
+        out["rates"][f"{rate:g}/s"] = srv.run(reqs)
+    return out
 
 
 def tgv_mixed_form(n: int, order: int, steps: int, warmup: int, tune: bool) -> dict:
@@ -351,11 +367,27 @@ def traffic_for(kernel: str, workload: str):
         return None
 
 
+def kernel_pass(s, path: int, steps: int) -> dict:
+    """Per-kernel average device times: `steps` more RK steps launched kernel
+    by kernel with HIP events recorded on the context's stream around every
+    launch (the captured graph of the timed region has no events inside),
+    read after the pass."""
+    s.enable_timing(True)
+    s.sync()
+    s.update(steps, path=path, sync=False)
+    s.sync()
+    kt = s.kernel_times()
+    s.enable_timing(False)
+    return kt
+
+
 def main() -> int:
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
-    ap.add_argument("--steps", type=int, default=50)
-    ap.add_argument("--warmup", type=int, default=3)
+    ap.add_argument("--steps", type=int, default=200)
+    ap.add_argument("--warmup", type=int, default=5)
+    ap.add_argument("--kernel-steps", type=int, default=20,
+                    help="RK steps of the per-kernel timing pass after the timed region")
     ap.add_argument("--n", type=int, default=48, help="elements per direction")
     ap.add_argument("--order", type=int, default=3)
     ap.add_argument("--inviscid", action="store_true")
@@ -375,8 +407,10 @@ def main() -> int:
     ap.add_argument("--batch-max-steps", type=int, default=3000)
     ap.add_argument("--batch-nx", type=int, default=128)
     ap.add_argument("--batch-ny", type=int, default=48)
+    ap.add_argument("--batch-cfl", type=str, default="0.9,0.9,0.9,0.9,0.9",
+                    help="local CFL number of orders 1..5, comma-separated")
     ap.add_argument("--serve-requests", type=int, default=24, help="0 skips the served form")
-    ap.add_argument("--serve-rate", type=float, default=20.0)
+    ap.add_argument("--serve-rates", type=str, default="5,20,80", help="Poisson arrival rates (per s), comma-separated")
     a = ap.parse_args()
 
     if a.gpus != 1:
@@ -410,15 +444,16 @@ def main() -> int:
     s.check_state()
     log("bench: warmup done")
 
-    s.enable_timing(True)
+    # timed region: the production mode, one captured HIP graph replayed per
+    # RK step (fused path), no events inside
     s.sync()
     t0 = time.perf_counter()
     s.update(a.steps, path=path, sync=False)
     s.sync()
     el = time.perf_counter() - t0
-    kt = s.kernel_times()
-    s.enable_timing(False)
     s.check_state()
+    U64 = s.get_U()
+    kt = kernel_pass(s, path, a.kernel_steps)
 
     nRK = setup.params.nRKSteps
     dof = setup.nEles * setup.nSpts
@@ -457,18 +492,15 @@ def main() -> int:
     if a.path == "fused" and not a.no_mixed:
         # mixed-precision form on the same case, from the same initial state
         from flurrypp_amd._native import PATH_MIXED
-        U64 = s.get_U()
         s.set_U(U)
         s.update(a.warmup, path=PATH_MIXED)
-        s.enable_timing(True)
         s.sync()
         t0 = time.perf_counter()
         s.update(a.steps, path=PATH_MIXED, sync=False)
         s.sync()
         elm = time.perf_counter() - t0
-        ktm = s.kernel_times()
-        s.enable_timing(False)
         U32 = s.get_U()
+        ktm = kernel_pass(s, PATH_MIXED, a.kernel_steps)
         rho_err = float(np.linalg.norm(U32[..., 0] - U64[..., 0]) / np.linalg.norm(U64[..., 0]))
         u_err = float(np.linalg.norm(U32 - U64) / np.linalg.norm(U64))
         T = (a.warmup + a.steps) * setup.params.dt
@@ -513,11 +545,13 @@ def main() -> int:
             a.mixed_n, a.order, a.steps, a.warmup, not a.no_tune))
     if a.batch_cases > 0:
         log(f"bench: batched form ({a.batch_cases} cases)")
-        forms["batched"] = optional("batched", lambda: batched_form(a.batch_cases, 11, a.batch_tol,
-                                                                     a.batch_max_steps, a.batch_nx, a.batch_ny))
+        forms["batched"] = optional("batched", lambda: batched_form(
+            a.batch_cases, 11, a.batch_tol, a.batch_max_steps, a.batch_nx, a.batch_ny,
+            [float(x) for x in a.batch_cfl.split(",")]))
     if a.serve_requests > 0:
         log(f"bench: served form ({a.serve_requests} requests)")
-        forms["served"] = optional("served", lambda: served_form(a.serve_requests, a.serve_rate, 13))
+        forms["served"] = optional("served", lambda: served_form(
+            a.serve_requests, [float(x) for x in a.serve_rates.split(",")], 13))
 
     cpu = None
     if not a.no_cpu:
@@ -535,6 +569,8 @@ def main() -> int:
         "config": {"workload": f"3D {'Navier-Stokes' if visc else 'Euler'} Taylor-Green vortex, {a.n}^3 periodic "
                                f"hex, P{a.order}, Rusanov{'+LDG' if visc else ''}, RK44, FP64",
                    "elements": setup.nEles, "dof": dof, "stages_per_step": nRK, "path": a.path,
+                   "timing": ("timed region replays one captured HIP graph per RK step; per-kernel averages "
+                              f"from a further {a.kernel_steps}-step pass with HIP events around every launch"),
                    "dt": setup.params.dt,
                    "block_sizes": tuned or "default 256 (extrapolate, face_inviscid, volume, face_boundary, rk)"},
         "roofline": roof,

@@ -25,7 +25,7 @@ ARCH = os.environ.get("FHIP_ARCH", "gfx950")
 
 # Packed-FP32 instructions are kept out of every device code object.
 HIP_FLAGS = [
-    "-O3", "-std=c++17", "-fPIC", "-shared", f"--offload-arch={ARCH}",
+    "-O3", "-std=c++17", "-fPIC", f"--offload-arch={ARCH}",
     "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops",
     "-munsafe-fp-atomics", "-Wno-unused-result",
 ]
@@ -70,13 +70,25 @@ def build_host() -> Path:
 
 
 def build_hip() -> Path:
+    """Each source to an object (in parallel: each holds its own kernels and
+    their launches), then one shared library."""
     LIB.mkdir(exist_ok=True)
     out = LIB / "libflurry_hip.so"
     dev = CSRC / "device"
-    srcs = [str(dev / f) for f in ("fhip_api.hip", "fhip_staged.hip", "fhip_fused.hip")]
-    tmp = out.with_suffix(".so.tmp")
+    obj = LIB / "obj"
+    obj.mkdir(exist_ok=True)
+    names = ("fhip_api", "fhip_staged", "fhip_fused")
     digest = source_hash("hip")
-    _run([HIPCC, *HIP_FLAGS, f"-I{INC}", f"-I{dev}", *srcs, "-o", str(tmp)])
+    procs = []
+    for n in names:
+        cmd = [HIPCC, *HIP_FLAGS, f"-I{INC}", f"-I{dev}", "-c", str(dev / f"{n}.hip"), "-o", str(obj / f"{n}.o")]
+        print("+", " ".join(cmd), flush=True)
+        procs.append(subprocess.Popen(cmd))
+    bad = [n for n, p in zip(names, procs) if p.wait() != 0]
+    if bad:
+        raise subprocess.CalledProcessError(1, f"hipcc {' '.join(bad)}")
+    tmp = out.with_suffix(".so.tmp")
+    _run([HIPCC, *HIP_FLAGS, "-shared", *(str(obj / f"{n}.o") for n in names), "-o", str(tmp)])
     os.replace(tmp, out)
     _stamp(out).write_text(digest + "\n")
     return out

@@ -122,11 +122,11 @@ def cfl_dt(setup: HostSetup, U: np.ndarray, cfl: float) -> float:
 
 
 # --------------------------------------------------------------------------
-def bump_mesh(nx: int, ny: int, jitter: float = 0.1, seed: int = SEED) -> Mesh:
-    """Channel [-1.5,1.5] x [y_b(x), 0.8], y_b = 0.0625 exp(-25 x^2);
+def bump_mesh(nx: int, ny: int, jitter: float = 0.1, seed: int = SEED, height: float = 0.0625) -> Mesh:
+    """Channel [-1.5,1.5] x [y_b(x), 0.8], y_b = height exp(-25 x^2);
     bottom/top slip walls, inlet/outlet characteristic."""
     xs = np.linspace(-1.5, 1.5, nx + 1)
-    yb = 0.0625 * np.exp(-25.0 * xs * xs)
+    yb = height * np.exp(-25.0 * xs * xs)
     j = np.arange(ny + 1)
     X = np.repeat(xs[None, :], ny + 1, 0)
     Y = yb[None, :] + (0.8 - yb[None, :]) * (j[:, None] / ny)
@@ -161,15 +161,17 @@ def _u01(x: int) -> float:
 
 
 def bump2d(nx: int = 32, ny: int = 12, order: int = 2, mach: float = 0.3, cfl: float = 0.5, dt_type: int = 2,
-           time_type: int = 4, jitter: float = 0.1, seed: int = SEED, iters: int = 100, dt: float = 1e-3):
+           time_type: int = 4, jitter: float = 0.1, seed: int = SEED, iters: int = 100, dt: float = 1e-3,
+           height: float = 0.0625):
     """Subsonic Euler flow over a Gaussian bump: rho=1, p=1/gamma (c=1),
     inlet velocity (M, 0); local-CFL time stepping by default (dt is used
-    with dt_type 0 only)."""
+    with dt_type 0 only).  height 0 gives a straight channel, in which the
+    free stream is an exact steady state."""
     g = 1.4
     p = _params(equation=1, nDims=2, order=order, meshType=0, meshFileName="-", timeType=time_type, dtType=dt_type,
                 CFL=cfl, dt=dt, maxTime=1e9, iterMax=iters, icType=0, riemannType=0, gamma=g, rhoBound=1.0, uBound=mach,
                 vBound=0.0, pBound=1.0 / g, mesh_bound={"fluid": "fluid"})
-    m = bump_mesh(nx, ny, jitter, seed)
+    m = bump_mesh(nx, ny, jitter, seed, height)
     setup = HostSetup(p, m)
     return setup, setup.initial_condition()
 

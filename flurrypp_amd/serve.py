@@ -42,6 +42,17 @@ class Request:
     dof: int = 0
     result: np.ndarray | None = None
     status: str = ""               # "converged", "max_steps", "diverged" or "done" (unsteady)
+    # mixed-precision steady requests finish in FP64 (see Server.promote_below):
+    # the step at which this one switched, or None
+    promoted_at: int | None = None
+    best: float = float("inf")     # lowest residual seen so far, and the check at which it was seen
+    best_check: int = 0
+    checks: int = 0
+
+    @property
+    def path(self) -> int:
+        """The path the request runs on now."""
+        return nat.PATH_MIXED if self.precision == "mixed" and self.promoted_at is None else nat.PATH_FUSED
 
 
 def build_case(r: Request):
@@ -93,6 +104,14 @@ class Server:
     check_every: int = 50          # steady requests: residual check interval (steps)
     keep_results: bool = False
     pmg: bool = False              # steady requests: P-multigrid down to order 0 after every step
+    # A mixed-precision steady request continues in FP64 once its residual is
+    # below promote_below, or once it has not halved over stall_checks residual
+    # checks: its fluxes and residual are FP32 sums, whose rounding floors
+    # the residual near 1e-7 (measured on the bump channels), above the 1e-8
+    # to 1e-10 tolerances steady requests ask for.  The FP64 state is kept
+    # by both paths, so the switch only changes the kernels.
+    promote_below: float = 1e-6
+    stall_checks: int = 8
     log: object = None
     active: list = field(default_factory=list)
     done: list = field(default_factory=list)
@@ -145,7 +164,7 @@ class Server:
             lib = nat.hip_lib()
             taken = {}
             for path in (nat.PATH_FUSED, nat.PATH_MIXED):
-                group = [r for r in self.active if (r.precision == "mixed") == (path == nat.PATH_MIXED)]
+                group = [r for r in self.active if r.path == path]
                 if not group:
                     continue
                 k = min([self.steps_per_tick] + [self._room(r) for r in group])
@@ -173,12 +192,21 @@ class Server:
                     if r.steps % self.check_every == 0 or r.steps >= r.max_steps:
                         res = s.residual_norm(2)
                         r.residual = float(res[0])
+                        r.checks += 1
+                        if r.residual < 0.5 * r.best:
+                            r.best, r.best_check = r.residual, r.checks
                         if not np.isfinite(res).all():
                             fin, r.status = True, "diverged"
                         elif r.residual <= r.steady_tol:
                             fin, r.status = True, "converged"
                         elif r.steps >= r.max_steps:
                             fin, r.status = True, "max_steps"
+                        elif r.path == nat.PATH_MIXED and (r.residual <= self.promote_below or
+                                                           r.checks - r.best_check >= self.stall_checks):
+                            r.promoted_at = r.steps
+                            if self.log is not None:
+                                self.log(f"request {r.rid} ({r.kind} P{r.order}) continues in FP64 after "
+                                         f"{r.steps} mixed steps, residual {r.residual:.3e}")
                 else:
                     fin = r.steps >= r.n_steps
                     if fin:
@@ -200,17 +228,33 @@ class Server:
                     still.append(r)
             self.active = still
         wall = now()
-        # time to solution of the requests that reached one (converged or finished their steps)
-        lat = np.array([r.finished - r.arrival for r in self.done if r.status in ("converged", "done")])
-        count = {k: sum(1 for r in self.done if r.status == k) for k in ("converged", "max_steps", "diverged", "done")}
-        return {
-            "requests": len(self.done), "status": count,
-            "wall_s": wall,
-            "p50_s": float(np.percentile(lat, 50)) if len(lat) else None,
-            "p99_s": float(np.percentile(lat, 99)) if len(lat) else None,
-            "throughput_req_per_s": len(self.done) / wall if wall > 0 else None,
-            "gdof_stage_per_s": total_dof_stages / wall / 1e9 if wall > 0 else None,
-        }
+        out = {"requests": len(self.done), "wall_s": wall}
+        out.update(latency_stats(self.done))
+        out["by_precision"] = {p: latency_stats([r for r in self.done if r.precision == p]) for p in ("fp64", "mixed")}
+        out["promoted_to_fp64"] = sum(1 for r in self.done if r.promoted_at is not None)
+        out["throughput_req_per_s"] = len(self.done) / wall if wall > 0 else None
+        out["gdof_stage_per_s"] = total_dof_stages / wall / 1e9 if wall > 0 else None
+        return out
+
+
+def _pct(x: list[float]) -> dict:
+    return {"p50_s": float(np.percentile(x, 50)) if len(x) else None,
+            "p99_s": float(np.percentile(x, 99)) if len(x) else None}
+
+
+def latency_stats(done: list[Request]) -> dict:
+    """Status counts and latencies (arrival to finish): p50/p99 over the
+    requests that reached a solution (converged steady, finished unsteady),
+    and separately over those that stopped without one (max_steps,
+    diverged), so that no request is left out of the report."""
+    count = {k: sum(1 for r in done if r.status == k) for k in ("converged", "max_steps", "diverged", "done")}
+    ok = [r.finished - r.arrival for r in done if r.status in ("converged", "done")]
+    bad = [r.finished - r.arrival for r in done if r.status in ("max_steps", "diverged")]
+    out = {"status": count}
+    out.update(_pct(ok))
+    out["unsolved"] = _pct(bad)
+    out["all"] = _pct(ok + bad)
+    return out
 
 
 def poisson_mix(n: int, rate: float, seed: int = 1, frac_2d: float = 0.7, steady_tol: float = 1e-6,
