@@ -157,6 +157,8 @@ def host_lib() -> C.CDLL:
         lib.fhost_mesh_get_nodes.argtypes = [C.c_void_p, _dp, _ip]
         lib.fhost_mesh_get_faces.argtypes = [C.c_void_p] + [_ip] * 8
         lib.fhost_mesh_geometry.argtypes = [C.c_void_p, C.c_int, C.c_int] + [_dp] * 6
+        lib.fhost_mesh_set_shape_nodes.argtypes = [C.c_void_p, C.c_int, _ip]
+        lib.fhost_mesh_get_shape_nodes.argtypes = [C.c_void_p, _ip, _ip]
         lib.fhost_mesh_destroy.argtypes = [C.c_void_p]
         lib.fhost_mesh_destroy.restype = None
         _host = lib

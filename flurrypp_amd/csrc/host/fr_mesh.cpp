@@ -1,4 +1,5 @@
-// Mesh, face connectivity and metric terms for linear quads and hexes.
+// Mesh, face connectivity and metric terms for quads and hexes (linear or
+// curved: Lagrange elements up to quintic, the 20-node serendipity hex).
 //
 // FlurryPP builds its face list with an O(n^2) search (matrix::unique,
 // src/matrix.cpp:668-695; findEq, src/geo.cpp:451; the periodic pairing loop,
@@ -65,8 +66,19 @@ struct fhost_mesh {
   std::vector<FaceRec> faces;
   int nIntFaces = 0, nBndFaces = 0;
 
+  // Geometry nodes (fhost_mesh_set_shape_nodes): nShape per element in
+  // Gmsh's node order, the first 4 (8) being the element's vertices; empty
+  // for linear elements, whose vertices are their nodes.  lattice[n] is node
+  // n's position (i, j, k) on the element's grid of nSide equidistant points
+  // per direction (Lagrange elements); the 20-node hexahedron is serendipity.
+  int nShape = 0, nSide = 2;
+  bool serendipity = false;
+  std::vector<int> c2n;
+  std::vector<std::array<int, 3>> lattice;
+
   const double* X(int v) const { return &xv[(size_t)v * nDims]; }
   double x(int v, int d) const { return xv[(size_t)v * nDims + d]; }
+  int node(size_t e, int n) const { return nShape ? c2n[e * nShape + n] : c2v[e * nNodes + n]; }
 };
 
 namespace {
@@ -420,31 +432,176 @@ void process(fhost_mesh* m) {
   m->processed = true;
 }
 
-// Linear Lagrange shape functions and derivatives on [-1,1] with FlurryPP's
-// corner order (src/funcs.cpp:145-196, 330-380, 382-460).
-void shape_lin(int D, const fr::Pt& p, double* s, double* ds /* [D][nNodes] */) {
-  static const std::vector<double> xl = {-1., 1.};
-  const int cx[8] = {0, 1, 1, 0, 0, 1, 1, 0}, cy[8] = {0, 0, 1, 1, 0, 0, 1, 1}, cz[8] = {0, 0, 0, 0, 1, 1, 1, 1};
-  int nn = (D == 2) ? 4 : 8;
-  for (int n = 0; n < nn; n++) {
-    double lx = fr::lagrange(xl, p.x, cx[n]), ly = fr::lagrange(xl, p.y, cy[n]);
-    double dx = fr::dlagrange(xl, p.x, cx[n]), dy = fr::dlagrange(xl, p.y, cy[n]);
-    if (D == 2) {
-      if (s) s[n] = lx * ly;
-      if (ds) {
-        ds[0 * nn + n] = dx * ly;
-        ds[1 * nn + n] = lx * dy;
+// ---- high-order geometry nodes ----------------------------------------------
+//
+// Gmsh numbers the nodes of a Lagrange quadrangle or hexahedron of n points
+// per edge as: its corners; the n-2 nodes inside each edge, walked from the
+// edge's first corner (hexahedron edges as corner pairs {0,1} {0,3} {0,4}
+// {1,2} {1,5} {2,3} {2,6} {3,7} {4,5} {4,7} {5,6} {6,7}); the nodes inside
+// each face, numbered as a quadrangle with n-2 points per edge whose corners
+// follow the face's corner list ({0,3,2,1} {0,1,5,4} {0,4,7,3} {1,2,6,5}
+// {2,3,7,6} {4,5,6,7}); then the interior, a hexahedron with n-2 points per
+// edge numbered the same way.  FlurryPP's shape functions use this order
+// (src/funcs.cpp:145-468).  Positions are lattice points (i, j, k), corner 0
+// at the origin and corner 6 at (n-1, n-1, n-1).
+using Lat = std::array<int, 3>;
+
+Lat lat_step(const Lat& a, const Lat& b, int num, int den) {  // a + (b - a) * num / den
+  Lat r;
+  for (int d = 0; d < 3; d++) r[d] = a[d] + (b[d] - a[d]) * num / den;
+  return r;
+}
+
+// Quadrangle with corners c[0..3] (lattice points) and n points per edge.
+void gmsh_quad(const Lat c[4], int n, std::vector<Lat>& out) {
+  if (n <= 0) return;
+  if (n == 1) {
+    out.push_back(c[0]);
+    return;
+  }
+  for (int k = 0; k < 4; k++) out.push_back(c[k]);
+  for (int k = 0; k < 4; k++)
+    for (int j = 1; j < n - 1; j++) out.push_back(lat_step(c[k], c[(k + 1) % 4], j, n - 1));
+  // the inner quadrangle: every corner one step along both of its edges
+  Lat in[4];
+  for (int k = 0; k < 4; k++) {
+    const Lat a = lat_step(c[k], c[(k + 1) % 4], 1, n - 1), b = lat_step(c[k], c[(k + 3) % 4], 1, n - 1);
+    for (int d = 0; d < 3; d++) in[k][d] = a[d] + b[d] - c[k][d];
+  }
+  gmsh_quad(in, n - 2, out);
+}
+
+void gmsh_hex(const Lat c[8], int n, std::vector<Lat>& out) {
+  static const int edges[12][2] = {{0, 1}, {0, 3}, {0, 4}, {1, 2}, {1, 5}, {2, 3},
+                                   {2, 6}, {3, 7}, {4, 5}, {4, 7}, {5, 6}, {6, 7}};
+  static const int faces[6][4] = {{0, 3, 2, 1}, {0, 1, 5, 4}, {0, 4, 7, 3}, {1, 2, 6, 5}, {2, 3, 7, 6}, {4, 5, 6, 7}};
+  if (n <= 0) return;
+  if (n == 1) {
+    out.push_back(c[0]);
+    return;
+  }
+  for (int k = 0; k < 8; k++) out.push_back(c[k]);
+  for (auto& e : edges)
+    for (int j = 1; j < n - 1; j++) out.push_back(lat_step(c[e[0]], c[e[1]], j, n - 1));
+  if (n > 2) {
+    for (auto& f : faces) {
+      Lat fc[4], in[4];
+      for (int k = 0; k < 4; k++) fc[k] = c[f[k]];
+      for (int k = 0; k < 4; k++) {
+        const Lat a = lat_step(fc[k], fc[(k + 1) % 4], 1, n - 1), b = lat_step(fc[k], fc[(k + 3) % 4], 1, n - 1);
+        for (int d = 0; d < 3; d++) in[k][d] = a[d] + b[d] - fc[k][d];
       }
-    } else {
-      double lz = fr::lagrange(xl, p.z, cz[n]), dz = fr::dlagrange(xl, p.z, cz[n]);
-      if (s) s[n] = lx * ly * lz;
-      if (ds) {
-        ds[0 * nn + n] = dx * ly * lz;
-        ds[1 * nn + n] = lx * dy * lz;
-        ds[2 * nn + n] = lx * ly * dz;
+      gmsh_quad(in, n - 2, out);
+    }
+    // the inner hexahedron: every corner one step inwards along its three edges
+    Lat in[8];
+    for (int k = 0; k < 8; k++)
+      for (int d = 0; d < 3; d++) in[k][d] = c[k][d] + (2 * c[k][d] < c[0][d] + c[6][d] ? 1 : -1);
+    gmsh_hex(in, n - 2, out);
+  }
+}
+
+std::vector<Lat> gmsh_lattice(int D, int n) {
+  std::vector<Lat> out;
+  const int h = n - 1;
+  if (D == 2) {
+    const Lat c[4] = {{0, 0, 0}, {h, 0, 0}, {h, h, 0}, {0, h, 0}};
+    gmsh_quad(c, n, out);
+  } else {
+    const Lat c[8] = {{0, 0, 0}, {h, 0, 0}, {h, h, 0}, {0, h, 0}, {0, 0, h}, {h, 0, h}, {h, h, h}, {0, h, h}};
+    gmsh_hex(c, n, out);
+  }
+  return out;
+}
+
+// Shape functions and their reference derivatives ds[D][nShape] of the
+// mesh's elements at p: Lagrange products on the equidistant grid
+// (src/funcs.cpp:145-468) or the 20-node serendipity hexahedron
+// (src/funcs.cpp:205-240; nodes in Gmsh order: corners, then the mid-points
+// of the edges in the edge order above).
+void shape_eval(const fhost_mesh* m, const fr::Pt& p, double* s, double* ds) {
+  const int D = m->nDims, NS = m->nShape ? m->nShape : m->nNodes;
+  const double q[3] = {p.x, p.y, p.z};
+  if (m->serendipity) {
+    static const int sg[8][3] = {{-1, -1, -1}, {1, -1, -1}, {1, 1, -1}, {-1, 1, -1},
+                                 {-1, -1, 1},  {1, -1, 1},  {1, 1, 1},  {-1, 1, 1}};
+    static const int edges[12][2] = {{0, 1}, {0, 3}, {0, 4}, {1, 2}, {1, 5}, {2, 3},
+                                     {2, 6}, {3, 7}, {4, 5}, {4, 7}, {5, 6}, {6, 7}};
+    for (int n = 0; n < 20; n++) {
+      double c[3];
+      for (int d = 0; d < 3; d++)
+        c[d] = n < 8 ? sg[n][d] : 0.5 * (sg[edges[n - 8][0]][d] + sg[edges[n - 8][1]][d]);
+      if (n < 8) {
+        // (1 + x c)(1 + y c)(1 + z c)(x cx + y cy + z cz - 2) / 8
+        double f[3], sum = -2;
+        for (int d = 0; d < 3; d++) f[d] = 1 + q[d] * c[d], sum += q[d] * c[d];
+        if (s) s[n] = 0.125 * f[0] * f[1] * f[2] * sum;
+        if (ds)
+          for (int d = 0; d < 3; d++) {
+            const int a = (d + 1) % 3, b = (d + 2) % 3;
+            ds[d * 20 + n] = 0.125 * f[a] * f[b] * (c[d] * sum + f[d] * c[d]);
+          }
+      } else {
+        // the coordinate with c = 0: (1 - x^2)(1 + y cy)(1 + z cz) / 4
+        const int z0 = c[0] == 0 ? 0 : (c[1] == 0 ? 1 : 2), a = (z0 + 1) % 3, b = (z0 + 2) % 3;
+        const double g = 1 - q[z0] * q[z0], fa = 1 + q[a] * c[a], fb = 1 + q[b] * c[b];
+        if (s) s[n] = 0.25 * g * fa * fb;
+        if (ds) {
+          ds[z0 * 20 + n] = 0.25 * (-2 * q[z0]) * fa * fb;
+          ds[a * 20 + n] = 0.25 * g * c[a] * fb;
+          ds[b * 20 + n] = 0.25 * g * fa * c[b];
+        }
       }
     }
+    return;
   }
+  const int n1 = m->nShape ? m->nSide : 2;
+  std::vector<double> xl(n1);
+  for (int i = 0; i < n1; i++) xl[i] = -1. + i * (2. / (n1 - 1));
+  double L[3][16], dL[3][16];
+  for (int d = 0; d < D; d++)
+    for (int i = 0; i < n1; i++) L[d][i] = fr::lagrange(xl, q[d], i), dL[d][i] = fr::dlagrange(xl, q[d], i);
+  static const int lin[8][3] = {{0, 0, 0}, {1, 0, 0}, {1, 1, 0}, {0, 1, 0}, {0, 0, 1}, {1, 0, 1}, {1, 1, 1}, {0, 1, 1}};
+  for (int n = 0; n < NS; n++) {
+    const int* ijk = m->nShape ? m->lattice[n].data() : lin[n];
+    double v = 1;
+    for (int d = 0; d < D; d++) v *= L[d][ijk[d]];
+    if (s) s[n] = v;
+    if (ds)
+      for (int r = 0; r < D; r++) {
+        double g = 1;
+        for (int d = 0; d < D; d++) g *= (d == r) ? dL[d][ijk[d]] : L[d][ijk[d]];
+        ds[r * NS + n] = g;
+      }
+  }
+}
+
+// Attach geometry nodes (Gmsh order) to a mesh: nShape per element.
+void set_shape_nodes(fhost_mesh* m, int nShape, const int* c2n) {
+  const int D = m->nDims;
+  int side = 0;
+  bool ser = false;
+  if (D == 3 && nShape == 20) {
+    ser = true;
+  } else {
+    for (int n = 2; n <= 6; n++)
+      if ((D == 2 ? n * n : n * n * n) == nShape) side = n;
+    if (!side)
+      throw Error(D == 2 ? "For Lagrange quad of order N, must have (N+1)^2 shape points."
+                         : "For Lagrange hex of order N, must have (N+1)^3 shape points (or the 20-node serendipity hex).");
+  }
+  for (size_t e = 0; e < (size_t)m->nEles; e++)
+    for (int n = 0; n < nShape; n++) {
+      const int v = c2n[e * nShape + n];
+      if (v < 0 || v >= m->nVerts) throw Error("shape node out of range");
+      if (n < m->nNodes && v != m->c2v[e * m->nNodes + n])
+        throw Error("the first shape nodes of an element must be its vertices");
+    }
+  m->nShape = nShape;
+  m->serendipity = ser;
+  m->nSide = ser ? 3 : side;
+  m->lattice = ser ? std::vector<Lat>() : gmsh_lattice(D, side);
+  m->c2n.assign(c2n, c2n + (size_t)m->nEles * nShape);
 }
 
 uint64_t splitmix64(uint64_t x) {
@@ -705,6 +862,24 @@ int fhost_mesh_from_arrays(int nDims, int nVerts, const double* xv, int nEles, c
   });
 }
 
+int fhost_mesh_set_shape_nodes(fhost_mesh* m, int nShape, const int* c2n) {
+  return guard([&] {
+    if (!m || !c2n) throw Error("null argument");
+    set_shape_nodes(m, nShape, c2n);
+  });
+}
+
+int fhost_mesh_get_shape_nodes(const fhost_mesh* m, int* nShape, int* c2n) {
+  return guard([&] {
+    if (!m) throw Error("null mesh");
+    const int ns = m->nShape ? m->nShape : m->nNodes;
+    if (nShape) *nShape = ns;
+    if (c2n)
+      for (size_t e = 0; e < (size_t)m->nEles; e++)
+        for (int n = 0; n < ns; n++) c2n[e * ns + n] = m->node(e, n);
+  });
+}
+
 int fhost_mesh_read_gmsh(const char* path, int nMap, const char* const* names, const char* const* bcs,
                          fhost_mesh** out) {
   return guard([&] {
@@ -782,7 +957,50 @@ int fhost_mesh_read_gmsh(const char* path, int nMap, const char* const* names, c
                                        std::to_string(v) + ", outside 1.." + std::to_string(nV));
       return v - 1;
     };
-    std::vector<int> c2v;
+    // element types (src/geo.cpp:1163-1344): Lagrange quadrangles of 4, 9,
+    // 16, 25, 36 nodes (Gmsh 3, 10, 36, 37, 38) and hexahedra of 8, 27, 64,
+    // 125, 216 nodes (5, 12, 92, 93, 94), the 20-node serendipity hexahedron
+    // (17); the 8-node quadrangle (16) is read, as FlurryPP reads it, and then
+    // refused, as its shape functions refuse it.  Boundary faces: lines of 2
+    // to 6 nodes (1, 8, 26, 27, 28) and quadrangles (3, 16, 10, 36, 37, 38);
+    // their corner vertices mark the boundary.
+    auto ele_nodes = [&](int type) {
+      switch (type) {
+        case 3: return nDims == 2 ? 4 : 0;
+        case 16: return nDims == 2 ? 8 : 0;
+        case 10: return nDims == 2 ? 9 : 0;
+        case 36: return nDims == 2 ? 16 : 0;
+        case 37: return nDims == 2 ? 25 : 0;
+        case 38: return nDims == 2 ? 36 : 0;
+        case 5: return nDims == 3 ? 8 : 0;
+        case 17: return nDims == 3 ? 20 : 0;
+        case 12: return nDims == 3 ? 27 : 0;
+        case 92: return nDims == 3 ? 64 : 0;
+        case 93: return nDims == 3 ? 125 : 0;
+        case 94: return nDims == 3 ? 216 : 0;
+        default: return 0;
+      }
+    };
+    auto face_nodes = [&](int type, int& corners) {
+      corners = nDims == 2 ? 2 : 4;
+      switch (type) {
+        case 1: return nDims == 2 ? 2 : 0;
+        case 8: return nDims == 2 ? 3 : 0;
+        case 26: return nDims == 2 ? 4 : 0;
+        case 27: return nDims == 2 ? 5 : 0;
+        case 28: return nDims == 2 ? 6 : 0;
+        case 3: return nDims == 3 ? 4 : 0;
+        case 16: return nDims == 3 ? 8 : 0;
+        case 10: return nDims == 3 ? 9 : 0;
+        case 36: return nDims == 3 ? 16 : 0;
+        case 37: return nDims == 3 ? 25 : 0;
+        case 38: return nDims == 3 ? 36 : 0;
+        default: return 0;
+      }
+    };
+    const int nv = (nDims == 2) ? 4 : 8;
+    std::vector<int> c2v, c2n;
+    int nShape = -1;
     std::vector<std::set<int>> bp(bcList.size());
     for (int k = 0; k < nE; k++) {
       int id, type, nTags, tag, tmp;
@@ -791,13 +1009,23 @@ int fhost_mesh_read_gmsh(const char* path, int nMap, const char* const* names, c
       if (!f) throw Error("malformed $Elements entry " + std::to_string(k + 1));
       int b = bcIdMap.count(tag) ? bcIdMap[tag] : -2;
       if (b == -1) {
-        int nn = (type == 3) ? 4 : (type == 5) ? 8 : 0;
-        if (!nn) throw Error("Gmsh element type " + std::to_string(type) + " not supported (linear quads/hexes only)");
-        for (int i = 0; i < nn; i++) c2v.push_back(vertex(k));
+        const int nn = ele_nodes(type);
+        if (!nn) throw Error("Gmsh element type " + std::to_string(type) + " not supported (quadrangles and hexahedra)");
+        if (nShape >= 0 && nn != nShape) throw Error("elements of different orders in one mesh are not supported");
+        nShape = nn;
+        for (int i = 0; i < nn; i++) {
+          const int v = vertex(k);
+          c2n.push_back(v);
+          if (i < nv) c2v.push_back(v);
+        }
       } else if (b >= 0) {
-        int np = (type == 1) ? 2 : (type == 3) ? 4 : 0;
+        int corners = 0;
+        const int np = face_nodes(type, corners);
         if (!np) throw Error("Boundary Element (Face) Type Not Recognized!");
-        for (int i = 0; i < np; i++) bp[b].insert(vertex(k));
+        for (int i = 0; i < np; i++) {
+          const int v = vertex(k);
+          if (i < corners) bp[b].insert(v);
+        }
       } else {
         throw Error("element tagged with an unknown physical id");
       }
@@ -808,15 +1036,24 @@ int fhost_mesh_read_gmsh(const char* path, int nMap, const char* const* names, c
       pts.insert(pts.end(), s.begin(), s.end());
       ptr.push_back((int)pts.size());
     }
-    int nn = (nDims == 2) ? 4 : 8;
-    if (fhost_mesh_from_arrays(nDims, nV, xv.data(), (int)(c2v.size() / nn), c2v.data(), (int)bcList.size(),
+    if (fhost_mesh_from_arrays(nDims, nV, xv.data(), (int)(c2v.size() / nv), c2v.data(), (int)bcList.size(),
                                bcList.data(), ptr.data(), pts.data(), out))
       throw Error(g_err);
+    if (nShape > nv) {
+      try {
+        set_shape_nodes(*out, nShape, c2n.data());
+      } catch (...) {
+        fhost_mesh_destroy(*out);
+        *out = nullptr;
+        throw;
+      }
+    }
   });
 }
 
 int fhost_mesh_jitter(fhost_mesh* m, const double* h, double amp, unsigned long long seed) {
   return guard([&] {
+    if (m->nShape) throw Error("jitter moves the vertices of linear meshes only (this mesh has curved elements)");
     // A vertex is interior if no boundary face (before periodic pairing)
     // touches it; find those faces with a count over all element faces.
     const int NF = m->nFacesPerEle;
@@ -899,7 +1136,7 @@ int fhost_mesh_get_faces(const fhost_mesh* m, int* eleL, int* locL, int* eleR, i
 int fhost_mesh_geometry(const fhost_mesh* m, int order, int ptsType, double* pos_spts, double* pos_fpts,
                         double* jginv_spts, double* detjac_spts, double* norm_fpts, double* da_fpts) {
   return guard([&] {
-    const int D = m->nDims, NN = m->nNodes;
+    const int D = m->nDims, NN = m->nShape ? m->nShape : m->nNodes;
     const size_t E = (size_t)m->nEles;
     int S, F, Ff;
     fr::sizes(D, order, S, F, Ff);
@@ -908,12 +1145,12 @@ int fhost_mesh_geometry(const fhost_mesh* m, int order, int ptsType, double* pos
     std::vector<double> sh_s((size_t)S * NN), sh_f((size_t)F * NN), ds_s((size_t)D * S * NN), ds_f((size_t)D * F * NN);
     std::vector<double> tmp((size_t)D * NN);
     for (int s = 0; s < S; s++) {
-      shape_lin(D, ls[s], &sh_s[(size_t)s * NN], tmp.data());
+      shape_eval(m, ls[s], &sh_s[(size_t)s * NN], tmp.data());
       for (int d = 0; d < D; d++)
         for (int n = 0; n < NN; n++) ds_s[((size_t)d * S + s) * NN + n] = tmp[d * NN + n];
     }
     for (int f = 0; f < F; f++) {
-      shape_lin(D, lf[f], &sh_f[(size_t)f * NN], tmp.data());
+      shape_eval(m, lf[f], &sh_f[(size_t)f * NN], tmp.data());
       for (int d = 0; d < D; d++)
         for (int n = 0; n < NN; n++) ds_f[((size_t)d * F + f) * NN + n] = tmp[d * NN + n];
     }
@@ -923,9 +1160,9 @@ int fhost_mesh_geometry(const fhost_mesh* m, int order, int ptsType, double* pos
     std::string err;
 #pragma omp parallel for schedule(static)
     for (long long e = 0; e < (long long)E; e++) {
-      double nodes[8][3];
+      std::vector<std::array<double, 3>> nodes(NN);
       for (int n = 0; n < NN; n++)
-        for (int d = 0; d < D; d++) nodes[n][d] = m->x(m->c2v[(size_t)e * NN + n], d);
+        for (int d = 0; d < D; d++) nodes[n][d] = m->x(m->node((size_t)e, n), d);
       // positions: shape * nodes (setPosSptsFpts, src/solver.cpp:1248-1313)
       for (int s = 0; s < S && pos_spts; s++)
         for (int d = 0; d < D; d++) {

@@ -129,7 +129,11 @@ class Mesh:
         return cls(h)
 
     @classmethod
-    def from_arrays(cls, xv: np.ndarray, c2v: np.ndarray, bounds: list[tuple[int, np.ndarray]]) -> "Mesh":
+    def from_arrays(cls, xv: np.ndarray, c2v: np.ndarray, bounds: list[tuple[int, np.ndarray]],
+                    shape_nodes: np.ndarray | None = None) -> "Mesh":
+        """Elements by vertex (c2v, FlurryPP corner order); shape_nodes, if
+        given, are the geometry nodes of curved elements [nEles][nShape] in
+        Gmsh's order (the first 4 or 8 being the vertices)."""
         xv = np.ascontiguousarray(xv, dtype=np.float64)
         c2v = np.ascontiguousarray(c2v, dtype=np.int32)
         nDims = xv.shape[1]
@@ -143,7 +147,11 @@ class Mesh:
         h = C.c_void_p()
         host_check(host_lib().fhost_mesh_from_arrays(nDims, xv.shape[0], dptr(xv), c2v.shape[0], iptr(c2v),
                                                       len(bounds), iptr(bc_list), iptr(ptr), iptr(pts_a), C.byref(h)))
-        return cls(h)
+        m = cls(h)
+        if shape_nodes is not None:
+            c2n = np.ascontiguousarray(shape_nodes, dtype=np.int32)
+            host_check(host_lib().fhost_mesh_set_shape_nodes(h, c2n.shape[1], iptr(c2n)))
+        return m
 
     @classmethod
     def read_gmsh(cls, path: str, mesh_bounds: dict[str, str]) -> "Mesh":
@@ -188,6 +196,15 @@ class Mesh:
         c2v = np.zeros((c["nEles"], 4 if c["nDims"] == 2 else 8), dtype=np.int32)
         host_lib().fhost_mesh_get_nodes(self._h, dptr(xv), iptr(c2v))
         return xv, c2v
+
+    def shape_nodes(self) -> np.ndarray:
+        """Geometry nodes of every element [nEles][nShape] (Gmsh order; the
+        vertices for linear elements)."""
+        ns = C.c_int()
+        host_check(host_lib().fhost_mesh_get_shape_nodes(self._h, C.byref(ns), None))
+        out = np.zeros((self.nEles, ns.value), dtype=np.int32)
+        host_check(host_lib().fhost_mesh_get_shape_nodes(self._h, None, iptr(out)))
+        return out
 
     def faces(self) -> Faces:
         if not self.processed:

@@ -100,11 +100,26 @@ int fhost_mesh_from_arrays(int nDims, int nVerts, const double* xv, int nEles,
                            const int* bnd_ptr, const int* bnd_pts,
                            fhost_mesh** out);
 
-/* Gmsh 2.2 ASCII reader for linear quads (type 3) and hexes (type 5) with
- * their boundary lines/quads (src/geo.cpp:1021-1475).  names[i] -> bcs[i]
- * maps physical names to BC names ("fluid" marks the interior). */
+/* Gmsh 2.2 ASCII reader (src/geo.cpp:1021-1475): Lagrange quadrangles of
+ * 4, 9, 16, 25 or 36 nodes (Gmsh types 3, 10, 36, 37, 38), Lagrange
+ * hexahedra of 8, 27, 64, 125 or 216 nodes (5, 12, 92, 93, 94) and the
+ * 20-node serendipity hexahedron (17), one element type per mesh, with their
+ * boundary lines/quadrangles of any of those orders.  The 8-node
+ * quadrangle (16) is refused, as FlurryPP's shape functions refuse it.
+ * names[i] -> bcs[i] maps physical names to BC names ("fluid" marks the
+ * interior). */
 int fhost_mesh_read_gmsh(const char* path, int nMap, const char* const* names,
                          const char* const* bcs, fhost_mesh** out);
+
+/* Geometry nodes of every element for curved (high-order) elements
+ * (ele::nodes, src/ele.cpp:405-430): c2n [nEles][nShape] in Gmsh's node
+ * order, nShape as for fhost_mesh_read_gmsh; the first 4 (8) must be the
+ * element's vertices.  The metric terms then come from these nodes' shape
+ * functions (src/funcs.cpp:145-468, calcTransforms src/solver.cpp:1587-1694).
+ * get returns nShape and, when c2n is not NULL, the nodes (the vertices for
+ * linear elements). */
+int fhost_mesh_set_shape_nodes(fhost_mesh* m, int nShape, const int* c2n);
+int fhost_mesh_get_shape_nodes(const fhost_mesh* m, int* nShape, int* c2n);
 
 /* Move every vertex that is not on a boundary by a uniform offset in
  * [-amp*h_d, +amp*h_d] per coordinate d (h_d: given spacing).  The offset of

@@ -4,6 +4,7 @@
 // Exit status 0 when every call succeeds.
 #include <cstdio>
 #include <string>
+#include <cmath>
 #include <vector>
 
 #include "flurry_host.h"
@@ -104,8 +105,52 @@ static int gmsh_case(const char* dir) {
   return 0;
 }
 
+// Curved elements: a 9-node quadrangle and a 27-node hexahedron at their
+// reference node positions (Gmsh order), metric terms at orders 0-5, the
+// shape nodes read back, and shape-node counts that must be refused.
+static int curved_case() {
+  const double q9[9][2] = {{-1, -1}, {1, -1}, {1, 1}, {-1, 1}, {0, -1}, {1, 0}, {0, 1}, {-1, 0}, {0, 0}};
+  const int ce[8][3] = {{-1, -1, -1}, {1, -1, -1}, {1, 1, -1}, {-1, 1, -1}, {-1, -1, 1}, {1, -1, 1}, {1, 1, 1}, {-1, 1, 1}};
+  const int ed[12][2] = {{0, 1}, {0, 3}, {0, 4}, {1, 2}, {1, 5}, {2, 3}, {2, 6}, {3, 7}, {4, 5}, {4, 7}, {5, 6}, {6, 7}};
+  const int fa[6][4] = {{0, 3, 2, 1}, {0, 1, 5, 4}, {0, 4, 7, 3}, {1, 2, 6, 5}, {2, 3, 7, 6}, {4, 5, 6, 7}};
+  std::vector<double> h27;
+  for (auto& c : ce)
+    for (int d = 0; d < 3; d++) h27.push_back(c[d]);
+  for (auto& e : ed)
+    for (int d = 0; d < 3; d++) h27.push_back(0.5 * (ce[e[0]][d] + ce[e[1]][d]));
+  for (auto& f : fa)
+    for (int d = 0; d < 3; d++) h27.push_back(0.25 * (ce[f[0]][d] + ce[f[1]][d] + ce[f[2]][d] + ce[f[3]][d]));
+  for (int d = 0; d < 3; d++) h27.push_back(0.0);
+  for (int D = 2; D <= 3; D++) {
+    const int nn = D == 2 ? 9 : 27, nc = D == 2 ? 4 : 8;
+    std::vector<int> c2v(nc), c2n(nn);
+    for (int i = 0; i < nc; i++) c2v[i] = i;
+    for (int i = 0; i < nn; i++) c2n[i] = i;
+    const int ptr[1] = {0};
+    fhost_mesh* m = nullptr;
+    CHK(fhost_mesh_from_arrays(D, nn, D == 2 ? &q9[0][0] : h27.data(), 1, c2v.data(), 0, nullptr, ptr, nullptr, &m));
+    if (fhost_mesh_set_shape_nodes(m, nn - 1, c2n.data()) == 0) return 1;  // not a Lagrange element
+    CHK(fhost_mesh_set_shape_nodes(m, nn, c2n.data()));
+    int ns = 0;
+    std::vector<int> back(nn);
+    CHK(fhost_mesh_get_shape_nodes(m, &ns, back.data()));
+    if (ns != nn || back != c2n) return 1;
+    for (int order = 0; order <= 5; order++) {
+      int S, F, Ff;
+      CHK(fhost_sizes(D, order, &S, &F, &Ff));
+      std::vector<double> ps((size_t)S * D), pf((size_t)F * D), jg((size_t)D * S * D), dj(S), nr((size_t)F * D), da(F);
+      CHK(fhost_mesh_geometry(m, order, 0, ps.data(), pf.data(), jg.data(), dj.data(), nr.data(), da.data()));
+      for (double v : dj)
+        if (std::fabs(v - 1.0) > 1e-12) return 1;
+    }
+    fhost_mesh_destroy(m);
+  }
+  return 0;
+}
+
 int main(int argc, char** argv) {
   if (operators_all()) return 1;
+  if (curved_case()) return 1;
   if (gmsh_case(argc > 1 ? argv[1] : "/tmp")) return 1;
   const char* per2[4] = {"periodic", "periodic", "periodic", "periodic"};
   const char* mix2[4] = {"slip_wall", "char", "sup_out", "sup_in"};

@@ -6,7 +6,7 @@ metric terms and the same boundary faces as the mesh built from the arrays it
 was written from.  FlurryPP's own linear test meshes (tests/euler/cylinder),
 when present, read into closed elements: every element's face normals times
 face areas, integrated over its faces, sum to zero, and every Jacobian is
-positive.  Quadratic (curved, 9-node) elements are outside this reader."""
+positive.  Curved (high-order) elements: tests/test_curved_mesh.py."""
 import re
 from pathlib import Path
 

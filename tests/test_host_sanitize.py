@@ -2,8 +2,8 @@
 (CPU only): tests/host_sanitize/driver.cpp calls every entry point of the
 host API (operators and P-multigrid operators at orders 0-5, Cartesian and
 array meshes in 2-D and 3-D with periodic and mixed boundaries, jitter, face
-lists, metric terms, a Gmsh file read back and a malformed one rejected) in
-a sanitized build."""
+lists, metric terms, curved 9-node quadrangles and 27-node hexahedra, a Gmsh
+file read back and a malformed one rejected) in a sanitized build."""
 import os
 import shutil
 import subprocess
