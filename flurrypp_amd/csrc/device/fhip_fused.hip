@@ -86,7 +86,34 @@ struct Tabs {
   const double* cu_w;   // [D][S][2]  their opp_correct_u entries
   // the same weights rounded to FP32 once, for the mixed path
   const float *ext_wf, *efn_wf, *grd_wf, *cor_wf, *cu_wf;
+  // opp_grad_spts as one 1-D derivative matrix d1[i][m] (derivative of basis
+  // m at point i) shared by every direction and line, FP64 and FP32; null
+  // where the operator is not of that form (the fused paths then do not take
+  // P3 hexes, whose FP64 volume kernel contracts with it on the matrix cores)
+  const double* d1;
+  const float* d1f;
 };
+
+// The 1-D contractions of FP64 P3 hexes along x and z run on the matrix
+// cores (v_mfma_f64_4x4x4f64: four independent 4x4x4 products per wave).  One
+// element per wave, lane L = solution point x + 4y + 16z.  The instruction
+// takes A[i][k] of block b from lane 16k + 4b + i, B[k][j] from lane
+// 16k + 4b + j, and returns D[i][j] = C[i][j] + sum_k A[i][k] B[k][j] in lane
+// 16i + 4b + j (measured lane layout).  With b = y:
+//   z: A = d1[L % 4][L / 16], B = the lane's own point: D is the z-derivative
+//      at the lane's own point;
+//   x: A = the point (x, z) = (L / 16, L % 4) of the lane's row y, B = the
+//      same operator entry: D is the x-derivative at the lane's own point.
+// C carries the terms added on the vector unit (the gradient's correction,
+// the divergence's other directions).  y stays on the vector unit.
+template <int D, int P, class R>
+constexpr bool kMfma = D == 3 && P == 3 && sizeof(R) == 8;
+
+__device__ __forceinline__ double mfma444(double a, double b, double c) {
+  return __builtin_amdgcn_mfma_f64_4x4x4f64(a, b, c, 0, 0, 0);
+}
+__device__ __forceinline__ float mfma444(float, float, float c) { return c; }  // never taken (kMfma)
+
 
 // Operator weights in the kernels' arithmetic type.
 template <class R>
@@ -400,6 +427,11 @@ __device__ __forceinline__ void volume_body(int E, const double* __restrict__ Ue
   const int sw = spt_lds(D, P1, s);
   const bool act = b < nb;
   const long long e = e0 + b;
+  // matrix-core contractions: this lane's operator entry d1[s % 4][s / 16]
+  // and the slab position of the point whose x and z are this lane's z and x
+  constexpr bool mf = kMfma<D, P, R>;
+  const int swx = mf ? spt_lds(D, P1, s / 16 + 4 * ((s / 4) % 4) + 16 * (s % 4)) : 0;
+  const R opv = mf ? tw<R>(tb.d1, tb.d1f)[(s % 4) * P1 + s / 16] : R(0);
   // the point's metric terms, loaded first so that their latency overlaps the
   // staging of the slab (and the gradient of viscous runs)
   R G[D][D] = {};
@@ -450,13 +482,18 @@ __device__ __forceinline__ void volume_body(int E, const double* __restrict__ Ue
         const R* cw = tw<R>(tb.cu_w, tb.cu_wf) + (d * S + s) * 2;
 #pragma unroll
         for (int k = 0; k < N; k++) {
-          R a = 0;
-#pragma unroll
-          for (int m = 0; m < P1; m++) a += gw[m] * sU[(b * N + k) * S + gi[m]];
           // the other correction entries of the row are exactly zero
           R c = 0;
 #pragma unroll
           for (int j = 0; j < 2; j++) c += cw[j] * sC[(b * N + k) * F + ci[j]];
+          if (mf && d != 1) {  // compile-time: d and mf are constants here
+            gr[d][k] = d == 2 ? mfma444(opv, sU[(b * N + k) * S + sw], c)
+                              : mfma444(sU[(b * N + k) * S + swx], opv, c);
+            continue;
+          }
+          R a = 0;
+#pragma unroll
+          for (int m = 0; m < P1; m++) a += gw[m] * sU[(b * N + k) * S + gi[m]];
           gr[d][k] = c + a;
         }
       }
@@ -579,18 +616,24 @@ __device__ __forceinline__ void volume_body(int E, const double* __restrict__ Ue
   }
   __syncthreads();
   if (act) {
-    // divergence (calcDivF_spts): sum over directions in order
+    // divergence (calcDivF_spts): sum over directions in order (matrix
+    // cores: y on the vector unit, then z and x added)
 #pragma unroll
     for (int k = 0; k < N; k++) {
       R tot = 0;
 #pragma unroll
       for (int d = 0; d < D; d++) {
+        if (mf && d != 1) continue;
         const int* gi = tb.grd_idx + (d * S + s) * P1;
         const R* gw = tw<R>(tb.grd_w, tb.grd_wf) + (d * S + s) * P1;
         R a = 0;
 #pragma unroll
         for (int m = 0; m < P1; m++) a += gw[m] * sF[((b * D + d) * N + k) * S + gi[m]];
-        tot = (d == 0) ? a : a + tot;
+        tot = (d == 0 || mf) ? a : a + tot;
+      }
+      if (mf) {
+        tot = mfma444(opv, sF[((b * D + 2) * N + k) * S + sw], tot);
+        tot = mfma444(sF[((b * D + 0) * N + k) * S + swx], opv, tot);
       }
       st_s<NT>(tot, &divFe[(e * N + k) * S + s]);
     }
@@ -640,6 +683,13 @@ __global__ __launch_bounds__(kMaxBlock) void k_volume(int E, const double* __res
                                                  const R* __restrict__ JGe, const R* __restrict__ iJe,
                                                  R* __restrict__ divFe, R* __restrict__ disFne, VFace<R> vf,
                                                  Tabs tb, Phys ph) {
+  volume_body<D, P, VISC, R, NT>(E, Ue, Us, dUce, JGe, iJe, divFe, disFne, vf, tb, ph);
+}
+template <int D, int P, bool VISC, class R, bool NT>
+__global__ __launch_bounds__(kMaxBlock) __attribute__((amdgpu_waves_per_eu(3))) void k_volume_w3(
+    int E, const double* __restrict__ Ue, const R* __restrict__ Us, const R* __restrict__ dUce, const R* __restrict__ JGe,
+    const R* __restrict__ iJe, R* __restrict__ divFe, R* __restrict__ disFne, VFace<R> vf, Tabs tb,
+    Phys ph) {
   volume_body<D, P, VISC, R, NT>(E, Ue, Us, dUce, JGe, iJe, divFe, disFne, vf, tb, ph);
 }
 template <int D, int P, bool VISC, class R, bool NT>
@@ -988,7 +1038,31 @@ static T* dal(size_t n) {
 
 // Extract the non-zero structure of the dense operators.  Returns false if an
 // operator row has more non-zeros than the tensor-product structure allows.
-static bool build_tables(const Ctx& c, const fhip_desc* d, std::vector<char>& blob, size_t off[15]) {
+// opp_grad_spts as one 1-D derivative matrix (Tabs::d1): true when every
+// entry equals its tensor-product counterpart d1[coordinate of q along d]
+// [coordinate of r along d] on q's d-line and is zero off it (the dense
+// operator of a tensor-product element on one 1-D point set is: its factors
+// in the other directions are exactly 0 or 1).
+static bool build_d1(const Ctx& c, const fhip_desc* dsc, std::vector<double>& d1) {
+  const int D = c.z.D, S = c.z.S, P1 = c.z.P + 1;
+  auto coord = [&](int q, int dd) { return dd == 0 ? q % P1 : (dd == 1 ? (q / P1) % P1 : q / (P1 * P1)); };
+  auto tangent = [&](int q, int dd) {  // the point's coordinates in the other directions
+    return dd == 0 ? q / P1 : (dd == 1 ? q % P1 + P1 * (q / (P1 * P1)) : q % (P1 * P1));
+  };
+  d1.assign(P1 * P1, 0.);
+  for (int i = 0; i < P1; i++)
+    for (int m = 0; m < P1; m++) d1[i * P1 + m] = dsc->opp_grad_spts[(size_t)i * S + m];  // d = 0, line 0
+  for (int dd = 0; dd < D; dd++)
+    for (int q = 0; q < S; q++)
+      for (int r = 0; r < S; r++) {
+        const double v = dsc->opp_grad_spts[((size_t)dd * S + q) * S + r];
+        const bool on = tangent(r, dd) == tangent(q, dd);
+        if (!(v == (on ? d1[coord(q, dd) * P1 + coord(r, dd)] : 0.))) return false;  // +0 == -0
+      }
+  return true;
+}
+
+static bool build_tables(const Ctx& c, const fhip_desc* d, std::vector<char>& blob, size_t off[17]) {
   const int D = c.z.D, S = c.z.S, F = c.z.F, P1 = c.z.P + 1, NC = 2 * D;
   std::vector<int> ext_idx((size_t)F * P1, 0), efn_dir(F, 0), grd_idx((size_t)D * S * P1, 0),
       cor_idx((size_t)S * NC, 0), cu_idx((size_t)D * S * 2, 0);
@@ -1078,6 +1152,15 @@ static bool build_tables(const Ctx& c, const fhip_desc* d, std::vector<char>& bl
   putf(grd_w, off[12]);
   putf(cor_w, off[13]);
   putf(cu_w, off[14]);
+  // the matrix-core path of FP64 P3 hexes needs the 1-D derivative matrix
+  std::vector<double> d1;
+  off[15] = off[16] = 0;
+  if (build_d1(c, d, d1)) {
+    put(d1.data(), d1.size() * 8, off[15]);
+    putf(d1, off[16]);
+  } else if (D == 3 && P1 == 4) {
+    return false;
+  }
   return true;
 }
 
@@ -1139,7 +1222,7 @@ bool fused_setup(Ctx& c, const fhip_desc* d) {
   const Sizes& z = c.z;
   if (z.P < 0 || z.P > 5) return false;
   std::vector<char> blob;
-  size_t off[15];
+  size_t off[17];
   if (!build_tables(c, d, blob, off)) return false;
   auto* st = new FusedState();
   c.fused = st;
@@ -1150,7 +1233,8 @@ bool fused_setup(Ctx& c, const fhip_desc* d) {
                 (const double*)(base + off[3]), (const int*)(base + off[4]),    (const double*)(base + off[5]),
                 (const int*)(base + off[6]),    (const double*)(base + off[7]), (const int*)(base + off[8]),
                 (const double*)(base + off[9]), (const float*)(base + off[10]), (const float*)(base + off[11]),
-                (const float*)(base + off[12]), (const float*)(base + off[13]), (const float*)(base + off[14])};
+                (const float*)(base + off[12]), (const float*)(base + off[13]), (const float*)(base + off[14]),
+                off[15] ? (const double*)(base + off[15]) : nullptr, off[16] ? (const float*)(base + off[16]) : nullptr};
   const long long E = z.E, S = z.S, F = z.F, N = z.N, D = z.D;
   st->stream = E * N * S * (long long)sizeof(double) > kStreamBytes;
   st->Ue = dal<double>(E * N * S);
@@ -1380,7 +1464,7 @@ struct Runner {
                          lds(kVolume, nt, VISC), c.stream, c.z.E, st.Ue, w.Us, w.dUce, w.JG, w.iJ, w.divFe, w.disFne,
                          vf, st.tb, c.ph);
     else
-      hipLaunchKernelGGL((k_volume<D, P, VISC, R, NT>), dim3(nblk(c.z.E, nt / S)), dim3(nt), lds(kVolume, nt, VISC),
+      hipLaunchKernelGGL((k_volume_w3<D, P, VISC, R, NT>), dim3(nblk(c.z.E, nt / S)), dim3(nt), lds(kVolume, nt, VISC),
                          c.stream, c.z.E, st.Ue, w.Us, w.dUce, w.JG, w.iJ, w.divFe, w.disFne, vf, st.tb, c.ph);
     FCHK(hipGetLastError());
   }
