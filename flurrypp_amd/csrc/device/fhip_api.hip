@@ -422,6 +422,35 @@ static bool resolve_path(fhip::Ctx& c, int path, bool& mixed) {
 // it are captured kernel by kernel rather than as graph launches.
 static thread_local bool t_capturing_cycle = false;
 
+// The launches of body() on stream st captured into an instantiated graph.
+// On any failure the capture is ended, nothing is kept and the error is
+// thrown: no half-built graph or executable is ever stored or launched.
+extern "C++" template <class F>
+static hipGraphExec_t capture(hipStream_t st, F&& body) {
+  hipGraph_t g = nullptr;
+  HCHK(hipStreamBeginCapture(st, hipStreamCaptureModeThreadLocal));
+  try {
+    body();
+  } catch (...) {
+    (void)hipStreamEndCapture(st, &g);
+    if (g) (void)hipGraphDestroy(g);
+    throw;
+  }
+  hipError_t e = hipStreamEndCapture(st, &g);
+  if (e != hipSuccess) {
+    if (g) (void)hipGraphDestroy(g);
+    HCHK(e);
+  }
+  hipGraphExec_t ge = nullptr;
+  e = hipGraphInstantiate(&ge, g, nullptr, nullptr, 0);
+  (void)hipGraphDestroy(g);
+  if (e != hipSuccess) {
+    if (ge) (void)hipGraphExecDestroy(ge);
+    HCHK(e);
+  }
+  return ge;
+}
+
 // One iteration of the main loop body: iter++, solver::update(src).  The
 // coarse P-multigrid levels run update(true) without advancing their counter.
 static void update_step(fhip::Ctx& c, bool fused, bool mixed, bool src = false, bool count = true) {
@@ -433,14 +462,7 @@ static void update_step(fhip::Ctx& c, bool fused, bool mixed, bool src = false, 
     // step and timed runs launch kernel by kernel.
     if (!refresh && !c.timing && c.use_graphs && !t_capturing_cycle) {
       hipGraphExec_t& ge = c.graph[src ? 1 : 0][mixed ? 1 : 0];
-      if (!ge) {
-        hipGraph_t g = nullptr;
-        HCHK(hipStreamBeginCapture(c.stream, hipStreamCaptureModeThreadLocal));
-        fhip::fused_rk_step(c, false, mixed);
-        HCHK(hipStreamEndCapture(c.stream, &g));
-        HCHK(hipGraphInstantiate(&ge, g, nullptr, nullptr, 0));
-        (void)hipGraphDestroy(g);
-      }
+      if (!ge) ge = capture(c.stream, [&] { fhip::fused_rk_step(c, false, mixed); });
       HCHK(hipGraphLaunch(ge, c.stream));
     } else {
       fhip::fused_rk_step(c, refresh, mixed);
@@ -505,7 +527,12 @@ struct fhip_pmg {
   std::vector<fhip_ctx*> lv;
   std::vector<double*> R, Pro, sol, corr;  // device; R[i]: i+1 -> i, Pro[i]: i -> i+1
   int smooth = 1;
-  hipEvent_t ev = nullptr;
+  // A cycle's levels run one after the other, so every launch of a cycle goes
+  // to the fine level's stream (the coarse levels' own streams are swapped
+  // for it while the cycle is queued).  Outside the cycle, each coarse
+  // level's stream and the fine stream are ordered both ways with one event
+  // per level, recorded and waited on outside any capture.
+  std::vector<hipEvent_t> ev;
   // one captured cycle per path, replayed on the fine level's stream; valid
   // while no level has re-allocated arrays since (graph_gen of every level)
   hipGraphExec_t graph[4] = {nullptr, nullptr, nullptr, nullptr};
@@ -522,7 +549,8 @@ static void pmg_free(fhip_pmg* mg) {
   for (auto* v : {&mg->R, &mg->Pro, &mg->sol, &mg->corr})
     for (double* p : *v)
       if (p) (void)hipFree(p);
-  if (mg->ev) (void)hipEventDestroy(mg->ev);
+  for (hipEvent_t e : mg->ev)
+    if (e) (void)hipEventDestroy(e);
   delete mg;
 }
 
@@ -559,7 +587,8 @@ int fhip_pmg_create(fhip_ctx* const* levels, int n_levels, const double* const* 
       mg->corr.push_back(dalloc<double>(nS));
       if (!c.src) c.src = dalloc<double>(nS);
     }
-    HCHK(hipEventCreateWithFlags(&mg->ev, hipEventDisableTiming));
+    mg->ev.assign(n_levels, nullptr);
+    for (auto& e : mg->ev) HCHK(hipEventCreateWithFlags(&e, hipEventDisableTiming));
     *out = mg;
     return 0;
   });
@@ -572,10 +601,33 @@ int fhip_pmg_create(fhip_ctx* const* levels, int n_levels, const double* const* 
 
 void fhip_pmg_destroy(fhip_pmg* mg) { pmg_free(mg); }
 
-// Orders work across two levels' streams.
-static void pmg_handoff(fhip_pmg* mg, fhip::Ctx& from, fhip::Ctx& to) {
-  HCHK(hipEventRecord(mg->ev, from.stream));
-  HCHK(hipStreamWaitEvent(to.stream, mg->ev, 0));
+// While a cycle is queued every level launches on the fine level's stream;
+// the coarse levels get their own streams back at scope exit.
+struct CycleStream {
+  fhip_pmg* mg;
+  std::vector<hipStream_t> own;
+  explicit CycleStream(fhip_pmg* m) : mg(m) {
+    hipStream_t fine = mg->lv.back()->c.stream;
+    for (fhip_ctx* h : mg->lv) {
+      own.push_back(h->c.stream);
+      h->c.stream = fine;
+    }
+  }
+  ~CycleStream() {
+    for (size_t i = 0; i < own.size(); i++) mg->lv[i]->c.stream = own[i];
+  }
+};
+
+// Work already queued on a coarse level's own stream comes before the cycle
+// (to == fine) or the cycle before later work there (to == coarse).
+static void pmg_join(fhip_pmg* mg, bool into_fine) {
+  const int n = (int)mg->lv.size();
+  hipStream_t fine = mg->lv[n - 1]->c.stream;
+  for (int i = 0; i + 1 < n; i++) {
+    hipStream_t coarse = mg->lv[i]->c.stream;
+    HCHK(hipEventRecord(mg->ev[i], into_fine ? coarse : fine));
+    HCHK(hipStreamWaitEvent(into_fine ? fine : coarse, mg->ev[i], 0));
+  }
 }
 
 static void pmg_residual(fhip::Ctx& c, bool fused, bool mixed) {
@@ -594,14 +646,14 @@ static void pmg_smooth(fhip::Ctx& c, int n, bool fused, bool mixed) {
 static void pmg_restrict(fhip_pmg* mg, int i) {
   fhip::Ctx& f = mg->lv[i + 1]->c;
   fhip::Ctx& c = mg->lv[i]->c;
-  pmg_handoff(mg, f, c);
   fhip::op_apply(c, "pmg_restrict", c.z.S, f.z.S, mg->R[i], f.U, c.U, false);
   fhip::op_apply(c, "pmg_restrict", c.z.S, f.z.S, mg->R[i], f.divF, c.divF, false);
 }
 
 // multiGrid::cycle (src/multigrid.cpp:169-313), P levels: the launches of one
-// cycle, on the levels' streams (joined back into the fine level's stream).
+// cycle, all on the fine level's stream (CycleStream).
 static void pmg_cycle_body(fhip_pmg* mg, const std::vector<char>& fused, const std::vector<char>& mixed) {
+  CycleStream one(mg);
   const int n = (int)mg->lv.size();
   fhip::Ctx& fine = mg->lv[n - 1]->c;
   {
@@ -628,7 +680,6 @@ static void pmg_cycle_body(fhip_pmg* mg, const std::vector<char>& fused, const s
       fhip::Ctx& up = mg->lv[i + 1]->c;
       pmg_smooth(g, mg->smooth, fused[i], mixed[i]);
       fhip::sub_array(g, mg->corr[i], g.U, mg->sol[i], g.z.nS());
-      pmg_handoff(mg, g, up);
       fhip::op_apply(up, "pmg_prolong", up.z.S, g.z.S, mg->Pro[i], mg->corr[i], up.U, true);
     }
   }
@@ -636,9 +687,9 @@ static void pmg_cycle_body(fhip_pmg* mg, const std::vector<char>& fused, const s
 
 // The first cycle on a path runs launch by launch (it also allocates what the
 // fused paths allocate on first use); the second is captured into one graph
-// across all levels' streams, which later cycles replay on the fine level's
-// stream.  Timed levels, disabled graphs or re-allocated arrays fall back to
-// launch-by-launch cycles (and a new capture).
+// on the fine level's stream, which later cycles replay there.  Timed levels,
+// disabled graphs or re-allocated arrays fall back to launch-by-launch cycles
+// (and a new capture).
 int fhip_pmg_cycle(fhip_pmg* mg, int path) {
   return guard([&] {
     require(mg, "null multigrid");
@@ -662,32 +713,24 @@ int fhip_pmg_cycle(fhip_pmg* mg, int path) {
       (void)hipGraphExecDestroy(ge);
       ge = nullptr;
     }
+    pmg_join(mg, true);
     if (!graphs || !mg->warm[path & 3]) {
       pmg_cycle_body(mg, fused, mixed);
       mg->warm[path & 3] = true;
+      pmg_join(mg, false);
       return 0;
     }
     if (!ge) {
-      hipGraph_t g = nullptr;
-      HCHK(hipStreamBeginCapture(fine.stream, hipStreamCaptureModeThreadLocal));
-      t_capturing_cycle = true;
-      try {
-        pmg_cycle_body(mg, fused, mixed);
-      } catch (...) {
-        t_capturing_cycle = false;
-        (void)hipStreamEndCapture(fine.stream, &g);
-        if (g) (void)hipGraphDestroy(g);
-        throw;
-      }
-      t_capturing_cycle = false;
-      HCHK(hipStreamEndCapture(fine.stream, &g));
-      hipError_t e = hipGraphInstantiate(&ge, g, nullptr, nullptr, 0);
-      (void)hipGraphDestroy(g);
-      HCHK(e);
+      struct Flag {  // t_capturing_cycle for the capture's duration, however it ends
+        Flag() { t_capturing_cycle = true; }
+        ~Flag() { t_capturing_cycle = false; }
+      } flag;
+      ge = capture(fine.stream, [&] { pmg_cycle_body(mg, fused, mixed); });
       for (int i = 0; i < n; i++) gen[i] = mg->lv[i]->c.graph_gen;
       mg->gen[path & 3] = gen;
     }
     HCHK(hipGraphLaunch(ge, fine.stream));
+    pmg_join(mg, false);
     return 0;
   });
 }
