@@ -180,7 +180,7 @@ def tgv_mixed_form(n: int, order: int, steps: int, warmup: int, tune: bool) -> d
     path's error against the FP64 one (density L2, kinetic-energy
     dissipation rate)."""
     from flurrypp_amd import cases
-    from flurrypp_amd._native import PATH_FUSED, PATH_MIXED
+    from flurrypp_amd._native import PATH_FUSED, PATH_MIXED, PATH_MIXED_BF16, PATH_MIXED_BF16X3
     from flurrypp_amd.solver import Solver
     t = time.perf_counter()
     setup, U = cases.tgv3d(n=n, order=order, viscous=1)
@@ -189,7 +189,10 @@ def tgv_mixed_form(n: int, order: int, steps: int, warmup: int, tune: bool) -> d
     host = time.perf_counter() - t
     dof, nRK = setup.nEles * setup.nSpts, setup.params.nRKSteps
     runs = {}
-    for name, path in (("fp64", PATH_FUSED), ("mixed", PATH_MIXED)):
+    paths = [("fp64", PATH_FUSED), ("mixed", PATH_MIXED)]
+    if order == 3:
+        paths += [("bf16_hi_lo", PATH_MIXED_BF16), ("bf16_hi_mid_lo", PATH_MIXED_BF16X3)]
+    for name, path in paths:
         s.set_U(U)
         if tune:
             s.autotune(path)
@@ -201,20 +204,28 @@ def tgv_mixed_form(n: int, order: int, steps: int, warmup: int, tune: bool) -> d
         runs[name] = (time.perf_counter() - t0, s.get_U())
         log(f"bench: TGV {n}^3 {name}: {dof * nRK * steps / runs[name][0] / 1e9:.3f} GDOF-stage/s")
     s.close()
-    (el64, U64), (el32, U32) = runs["fp64"], runs["mixed"]
+    el64, U64 = runs["fp64"]
     T = (warmup + steps) * setup.params.dt
-    ek0, ek64, ek32 = (kinetic_energy(setup, x) for x in (U, U64, U32))
-    eps64, eps32 = -(ek64 - ek0) / T, -(ek32 - ek0) / T
-    return {
-        "workload": f"3D Navier-Stokes Taylor-Green vortex, {n}^3 periodic hex, P{order}, Rusanov+LDG, RK44",
-        "elements": setup.nEles, "dof": dof, "host_setup_s": round(host, 2),
-        "value": round(dof * nRK * steps / el32 / 1e9, 4), "unit": "GDOF·stage/s",
-        "fp64_value": round(dof * nRK * steps / el64 / 1e9, 4), "speedup_vs_fp64": round(el64 / el32, 3),
-        "rel_l2_density_vs_fp64": float(np.linalg.norm(U32[..., 0] - U64[..., 0]) / np.linalg.norm(U64[..., 0])),
-        "rel_l2_state_vs_fp64": float(np.linalg.norm(U32 - U64) / np.linalg.norm(U64)),
-        "ke_dissipation_rel_err": abs(eps32 - eps64) / abs(eps64) if eps64 else None,
-        "after_steps": warmup + steps,
-    }
+    ek0, ek64 = kinetic_energy(setup, U), kinetic_energy(setup, U64)
+    eps64 = -(ek64 - ek0) / T
+
+    def against_fp64(name):
+        el, V = runs[name]
+        eps = -(kinetic_energy(setup, V) - ek0) / T
+        return {"value": round(dof * nRK * steps / el / 1e9, 4), "speedup_vs_fp64": round(el64 / el, 3),
+                "rel_l2_density_vs_fp64": float(np.linalg.norm(V[..., 0] - U64[..., 0]) / np.linalg.norm(U64[..., 0])),
+                "rel_l2_state_vs_fp64": float(np.linalg.norm(V - U64) / np.linalg.norm(U64)),
+                "ke_dissipation_rel_err": abs(eps - eps64) / abs(eps64) if eps64 else None}
+    out = {"workload": f"3D Navier-Stokes Taylor-Green vortex, {n}^3 periodic hex, P{order}, Rusanov+LDG, RK44",
+           "elements": setup.nEles, "dof": dof, "host_setup_s": round(host, 2), "unit": "GDOF·stage/s",
+           "fp64_value": round(dof * nRK * steps / el64 / 1e9, 4), "after_steps": warmup + steps}
+    out.update(against_fp64("mixed"))
+    # the operators' z contractions as BF16 pieces on the matrix cores (FP32
+    # accumulation), beside the FP32 vector-unit form above
+    for name in ("bf16_hi_lo", "bf16_hi_mid_lo"):
+        if name in runs:
+            out[name] = against_fp64(name)
+    return out
 
 
 def vortex2d_form(steps: int, tune: bool) -> dict:

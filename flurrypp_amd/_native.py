@@ -56,6 +56,8 @@ ARR = {
     "face_fpt_L": 12, "face_fpt_R": 13,
 }
 PATH_AUTO, PATH_STAGED, PATH_FUSED, PATH_MIXED = 0, 1, 2, 3
+# mixed precision with the z contractions of P3 hexes as BF16 hi+lo (hi+mid+lo) matrix-core products
+PATH_MIXED_BF16, PATH_MIXED_BF16X3 = 4, 5
 
 
 def dptr(a: np.ndarray | None):

@@ -409,9 +409,11 @@ int fhip_get_iter(fhip_ctx* h, int* iter) {
 }
 
 static bool resolve_path(fhip::Ctx& c, int path, bool& mixed) {
-  require(path == FHIP_PATH_AUTO || path == FHIP_PATH_STAGED || path == FHIP_PATH_FUSED || path == FHIP_PATH_MIXED,
-          "unknown path");
-  mixed = path == FHIP_PATH_MIXED;
+  require(path >= FHIP_PATH_AUTO && path <= FHIP_PATH_MIXED_BF16X3, "unknown path");
+  const bool bf = path == FHIP_PATH_MIXED_BF16 || path == FHIP_PATH_MIXED_BF16X3;
+  require(!bf || (c.z.D == 3 && c.z.P == 3), "the BF16 matrix-core forms cover P3 hexes only");
+  c.bf16 = path == FHIP_PATH_MIXED_BF16 ? 2 : (path == FHIP_PATH_MIXED_BF16X3 ? 3 : 0);
+  mixed = path == FHIP_PATH_MIXED || bf;
   bool fused = mixed || (path == FHIP_PATH_FUSED) || (path == FHIP_PATH_AUTO && c.fused_ok);
   require(!fused || c.fused_ok, "the fused paths do not support this configuration");
   require(fused || !c.batched, "batched contexts (several meshes or a boundary-state table) need a fused path");
@@ -461,7 +463,7 @@ static void update_step(fhip::Ctx& c, bool fused, bool mixed, bool src = false, 
     // Steady-state steps replay one captured graph; the wave-speed refresh
     // step and timed runs launch kernel by kernel.
     if (!refresh && !c.timing && c.use_graphs && !t_capturing_cycle) {
-      hipGraphExec_t& ge = c.graph[src ? 1 : 0][mixed ? 1 : 0];
+      hipGraphExec_t& ge = c.graph[src ? 1 : 0][mixed ? 1 + (c.bf16 ? c.bf16 - 1 : 0) : 0];
       if (!ge) ge = capture(c.stream, [&] { fhip::fused_rk_step(c, false, mixed); });
       HCHK(hipGraphLaunch(ge, c.stream));
     } else {
@@ -529,15 +531,13 @@ struct fhip_pmg {
   int smooth = 1;
   // A cycle's levels run one after the other, so every launch of a cycle goes
   // to the fine level's stream (the coarse levels' own streams are swapped
-  // for it while the cycle is queued).  Outside the cycle, each coarse
-  // level's stream and the fine stream are ordered both ways with one event
-  // per level, recorded and waited on outside any capture.
-  std::vector<hipEvent_t> ev;
+  // for it while the cycle is queued); work a caller queued on a coarse
+  // level's own stream is finished before a cycle is queued.
   // one captured cycle per path, replayed on the fine level's stream; valid
   // while no level has re-allocated arrays since (graph_gen of every level)
-  hipGraphExec_t graph[4] = {nullptr, nullptr, nullptr, nullptr};
-  std::vector<unsigned long long> gen[4];
-  bool warm[4] = {false, false, false, false};
+  hipGraphExec_t graph[6] = {};  // by path
+  std::vector<unsigned long long> gen[6];
+  bool warm[6] = {};
 };
 
 static void pmg_free(fhip_pmg* mg) {
@@ -549,8 +549,6 @@ static void pmg_free(fhip_pmg* mg) {
   for (auto* v : {&mg->R, &mg->Pro, &mg->sol, &mg->corr})
     for (double* p : *v)
       if (p) (void)hipFree(p);
-  for (hipEvent_t e : mg->ev)
-    if (e) (void)hipEventDestroy(e);
   delete mg;
 }
 
@@ -587,8 +585,6 @@ int fhip_pmg_create(fhip_ctx* const* levels, int n_levels, const double* const* 
       mg->corr.push_back(dalloc<double>(nS));
       if (!c.src) c.src = dalloc<double>(nS);
     }
-    mg->ev.assign(n_levels, nullptr);
-    for (auto& e : mg->ev) HCHK(hipEventCreateWithFlags(&e, hipEventDisableTiming));
     *out = mg;
     return 0;
   });
@@ -618,15 +614,18 @@ struct CycleStream {
   }
 };
 
-// Work already queued on a coarse level's own stream comes before the cycle
-// (to == fine) or the cycle before later work there (to == coarse).
-static void pmg_join(fhip_pmg* mg, bool into_fine) {
+// Work a caller queued on a coarse level's own stream (set-up, a residual)
+// is finished on the host before a cycle is queued: cycles leave nothing
+// there, so this is rare, and no device-side wait ties the fine stream to the
+// coarse ones (a wait can hold back other contexts' streams that share its
+// hardware queue).
+static void pmg_settle_coarse(fhip_pmg* mg) {
   const int n = (int)mg->lv.size();
-  hipStream_t fine = mg->lv[n - 1]->c.stream;
   for (int i = 0; i + 1 < n; i++) {
     hipStream_t coarse = mg->lv[i]->c.stream;
-    HCHK(hipEventRecord(mg->ev[i], into_fine ? coarse : fine));
-    HCHK(hipStreamWaitEvent(into_fine ? fine : coarse, mg->ev[i], 0));
+    const hipError_t q = hipStreamQuery(coarse);
+    if (q == hipErrorNotReady) HCHK(hipStreamSynchronize(coarse));
+    else HCHK(q);
   }
 }
 
@@ -707,17 +706,16 @@ int fhip_pmg_cycle(fhip_pmg* mg, int path) {
     }
     fhip::Ctx& fine = mg->lv[n - 1]->c;
     HCHK(hipSetDevice(fine.device));
-    hipGraphExec_t& ge = mg->graph[path & 3];
-    if (ge && (!graphs || mg->gen[path & 3] != gen)) {
+    hipGraphExec_t& ge = mg->graph[path];
+    if (ge && (!graphs || mg->gen[path] != gen)) {
       HCHK(hipStreamSynchronize(fine.stream));
       (void)hipGraphExecDestroy(ge);
       ge = nullptr;
     }
-    pmg_join(mg, true);
-    if (!graphs || !mg->warm[path & 3]) {
+    pmg_settle_coarse(mg);
+    if (!graphs || !mg->warm[path]) {
       pmg_cycle_body(mg, fused, mixed);
-      mg->warm[path & 3] = true;
-      pmg_join(mg, false);
+      mg->warm[path] = true;
       return 0;
     }
     if (!ge) {
@@ -727,10 +725,9 @@ int fhip_pmg_cycle(fhip_pmg* mg, int path) {
       } flag;
       ge = capture(fine.stream, [&] { pmg_cycle_body(mg, fused, mixed); });
       for (int i = 0; i < n; i++) gen[i] = mg->lv[i]->c.graph_gen;
-      mg->gen[path & 3] = gen;
+      mg->gen[path] = gen;
     }
     HCHK(hipGraphLaunch(ge, fine.stream));
-    pmg_join(mg, false);
     return 0;
   });
 }
@@ -758,6 +755,8 @@ int fhip_path_supported(fhip_ctx* h, int path) {
   if (!h) return 0;
   if (path == FHIP_PATH_STAGED || path == FHIP_PATH_AUTO) return 1;
   if (path == FHIP_PATH_FUSED || path == FHIP_PATH_MIXED) return h->c.fused_ok ? 1 : 0;
+  if (path == FHIP_PATH_MIXED_BF16 || path == FHIP_PATH_MIXED_BF16X3)
+    return h->c.fused_ok && h->c.z.D == 3 && h->c.z.P == 3 ? 1 : 0;
   return 0;
 }
 

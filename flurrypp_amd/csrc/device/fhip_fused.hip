@@ -114,6 +114,84 @@ __device__ __forceinline__ double mfma444(double a, double b, double c) {
 }
 __device__ __forceinline__ float mfma444(float, float, float c) { return c; }  // never taken (kMfma)
 
+// The mixed path's z contractions of P3 hexes on the matrix cores in BF16
+// (v_mfma_f32_16x16x32_bf16, FP32 accumulation).  Both factors are split into
+// BF16 pieces: x = p0 + p1 + p2 with p0 = bf16(x), p1 = bf16(x - p0),
+// p2 = bf16(x - p0 - p1).  With 2 pieces (hi+lo) the product takes the terms
+// d0 v0 + d0 v1 + d1 v0, with 3 also d0 v2 + d2 v0 + d1 v1 (the rest are
+// below FP32 rounding).  One element per wave, lane L = point xy + 16 z
+// (xy = L % 16).  The instruction takes A[i][k] from lane 16(k/8) + i, element
+// k % 8, B[k][j] from lane 16(k/8) + j, element k % 8, and returns D[i][j] in
+// lane 16(i/4) + j, register i % 4 (measured lane layout).  With j = xy, the
+// rows i = 4z of A hold the pieces of d1[z][m] and the column xy of B the
+// pieces of v(xy, m), m = 0..3, eight K slots per lane group g = L / 16:
+//   2 pieces: g0: (d0 | d0) x (v0 | v1), g1: (d1 | 0) x (v0 | 0)
+//   3 pieces: g0: (d0 | d0) x (v0 | v1), g1: (d1 | d0) x (v0 | v2),
+//             g2: (d2 | d1) x (v0 | v1)
+// so register 0 of each lane is the z contraction at the lane's own point.
+__host__ __device__ constexpr int spt_lds(int D, int P1, int s);  // the slab position of point s (below)
+
+typedef __bf16 bf16x8 __attribute__((ext_vector_type(8)));
+typedef float f32x4 __attribute__((ext_vector_type(4)));
+
+__device__ __forceinline__ void bf16_pieces(float x, __bf16 p[3]) {
+  p[0] = (__bf16)x;
+  const float r1 = x - float(p[0]);
+  p[1] = (__bf16)r1;
+  p[2] = (__bf16)(r1 - float(p[1]));
+}
+
+// Which pieces lane group g puts in its two halves of eight K slots (-1: zero).
+__device__ __forceinline__ void bf16_plan(int np, int g, int& da, int& db, int& va, int& vb) {
+  da = db = va = vb = -1;
+  if (g == 0) da = 0, db = 0, va = 0, vb = 1;
+  if (g == 1) da = 1, db = (np == 3 ? 0 : -1), va = 0, vb = (np == 3 ? 2 : -1);
+  if (g == 2 && np == 3) da = 2, db = 1, va = 0, vb = 1;
+}
+
+// A operand of lane L: the pieces of row z = (L % 16) / 4 of d1 for rows
+// i = L % 16 with i % 4 == 0, else zero.
+__device__ __forceinline__ bf16x8 bf16_opnd_a(const float* d1, int lane, int np) {
+  bf16x8 a;
+  for (int e = 0; e < 8; e++) a[e] = (__bf16)0.f;
+  const int i = lane % 16, g = lane / 16;
+  if (i % 4 != 0) return a;
+  int da, db, va, vb;
+  bf16_plan(np, g, da, db, va, vb);
+  for (int m = 0; m < 4; m++) {
+    __bf16 p[3];
+    bf16_pieces(d1[(i / 4) * 4 + m], p);
+    if (da >= 0) a[m] = p[da];
+    if (db >= 0) a[4 + m] = p[db];
+  }
+  return a;
+}
+
+// The z contraction at the lane's own point plus c: v is this field's slab
+// (swizzled LDS positions), column xy = lane % 16.
+template <int D, int P1>
+__device__ __forceinline__ float mfma_z_bf16(const float* v, int lane, bf16x8 a, float c, int np) {
+  const int xy = lane % 16, g = lane / 16;
+  int da, db, va, vb;
+  bf16_plan(np, g, da, db, va, vb);
+  bf16x8 bv;
+  for (int e = 0; e < 8; e++) bv[e] = (__bf16)0.f;
+  if (va >= 0) {
+    for (int m = 0; m < 4; m++) {
+      __bf16 p[3];
+      bf16_pieces(v[spt_lds(D, P1, xy + 16 * m)], p);
+      bv[m] = p[va];
+      if (vb >= 0) bv[4 + m] = p[vb];
+    }
+  }
+  const f32x4 cc = {c, 0.f, 0.f, 0.f};
+  const f32x4 d = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a, bv, cc, 0, 0, 0);
+  return d[0];
+}
+
+template <int D, int P, class R>
+constexpr bool kBf16 = D == 3 && P == 3 && sizeof(R) == 4;
+
 
 // Operator weights in the kernels' arithmetic type.
 template <class R>
@@ -387,7 +465,7 @@ template <class R>
 constexpr bool kJumpInVolume = sizeof(R) == 4;
 
 // Shapes whose volume kernel is held to 4 waves per SIMD (128 VGPRs): quads
-// and the FP32 path.  FP64 hexes need more registers and run unheld.
+// and the FP32 path.  FP64 hexes need more registers and are held to 3.
 template <int D, class R>
 constexpr bool kHold = D == 2 || sizeof(R) == 4;
 
@@ -407,7 +485,7 @@ struct VFace {
 // Volume kernel: [corrected gradient], flux, divergence, normal flux at faces
 // [and gradient at faces].  One thread per solution point, EB elements per
 // block, element data staged in LDS.
-template <int D, int P, bool VISC, class R, bool NT>
+template <int D, int P, bool VISC, class R, bool NT, int BF = 0>
 __device__ __forceinline__ void volume_body(int E, const double* __restrict__ Ue, const R* __restrict__ Us,
                                             const R* __restrict__ dUce,
                                             const R* __restrict__ JGe, const R* __restrict__ iJe,
@@ -430,6 +508,11 @@ __device__ __forceinline__ void volume_body(int E, const double* __restrict__ Ue
   // matrix-core contractions: this lane's operator entry d1[s % 4][s / 16]
   // and the slab position of the point whose x and z are this lane's z and x
   constexpr bool mf = kMfma<D, P, R>;
+  // mixed path, P3 hexes (BF = 2 or 3): z contractions in BF16 pieces on the
+  // matrix cores
+  constexpr int np = kBf16<D, P, R> ? BF : 0;
+  bf16x8 abf;
+  if constexpr (np != 0) abf = bf16_opnd_a(tw<float>(tb.d1, tb.d1f), s, np);
   const int swx = mf ? spt_lds(D, P1, s / 16 + 4 * ((s / 4) % 4) + 16 * (s % 4)) : 0;
   const R opv = mf ? tw<R>(tb.d1, tb.d1f)[(s % 4) * P1 + s / 16] : R(0);
   // the point's metric terms, loaded first so that their latency overlaps the
@@ -486,6 +569,12 @@ __device__ __forceinline__ void volume_body(int E, const double* __restrict__ Ue
           R c = 0;
 #pragma unroll
           for (int j = 0; j < 2; j++) c += cw[j] * sC[(b * N + k) * F + ci[j]];
+          if constexpr (np != 0) {
+            if (d == 2) {
+              gr[d][k] = mfma_z_bf16<D, P1>(reinterpret_cast<const float*>(sU + (b * N + k) * S), s, abf, c, np);
+              continue;
+            }
+          }
           if (mf && d != 1) {  // compile-time: d and mf are constants here
             gr[d][k] = d == 2 ? mfma444(opv, sU[(b * N + k) * S + sw], c)
                               : mfma444(sU[(b * N + k) * S + swx], opv, c);
@@ -624,6 +713,12 @@ __device__ __forceinline__ void volume_body(int E, const double* __restrict__ Ue
 #pragma unroll
       for (int d = 0; d < D; d++) {
         if (mf && d != 1) continue;
+        if constexpr (np != 0) {
+          if (d == 2) {
+            tot = mfma_z_bf16<D, P1>(reinterpret_cast<const float*>(sF + ((b * D + 2) * N + k) * S), s, abf, tot, np);
+            continue;
+          }
+        }
         const int* gi = tb.grd_idx + (d * S + s) * P1;
         const R* gw = tw<R>(tb.grd_w, tb.grd_wf) + (d * S + s) * P1;
         R a = 0;
@@ -671,20 +766,12 @@ __device__ __forceinline__ void volume_body(int E, const double* __restrict__ Ue
   }
 }
 
-// The volume kernel, and the same kernel held to 4 waves per SIMD (at most
-// 128 VGPRs) for quads and the FP32 path, which fit there without spilling
-// (holding the FP32 viscous hex kernel to 5 or 6 waves spills 26 or 41 VGPRs
-// and was measured 13-16% slower).  Under the hold, viscous runs reload the
-// metric terms after the face loop; inviscid runs keep them in registers
-// throughout.
-template <int D, int P, bool VISC, class R, bool NT>
-__global__ __launch_bounds__(kMaxBlock) void k_volume(int E, const double* __restrict__ Ue, const R* __restrict__ Us,
-                                                 const R* __restrict__ dUce,
-                                                 const R* __restrict__ JGe, const R* __restrict__ iJe,
-                                                 R* __restrict__ divFe, R* __restrict__ disFne, VFace<R> vf,
-                                                 Tabs tb, Phys ph) {
-  volume_body<D, P, VISC, R, NT>(E, Ue, Us, dUce, JGe, iJe, divFe, disFne, vf, tb, ph);
-}
+// The volume kernel held to 4 waves per SIMD (at most 128 VGPRs) for quads
+// and the FP32 path, which fit there without spilling (holding the FP32
+// viscous hex kernel to 5 or 6 waves spills 26 or 41 VGPRs and was measured
+// 13-16% slower), and to 3 waves (168 VGPRs) for FP64 hexes.  Viscous runs
+// reload the metric terms after the face loop; inviscid runs keep them in
+// registers throughout.
 template <int D, int P, bool VISC, class R, bool NT>
 __global__ __launch_bounds__(kMaxBlock) __attribute__((amdgpu_waves_per_eu(3))) void k_volume_w3(
     int E, const double* __restrict__ Ue, const R* __restrict__ Us, const R* __restrict__ dUce, const R* __restrict__ JGe,
@@ -692,12 +779,12 @@ __global__ __launch_bounds__(kMaxBlock) __attribute__((amdgpu_waves_per_eu(3))) 
     Phys ph) {
   volume_body<D, P, VISC, R, NT>(E, Ue, Us, dUce, JGe, iJe, divFe, disFne, vf, tb, ph);
 }
-template <int D, int P, bool VISC, class R, bool NT>
+template <int D, int P, bool VISC, class R, bool NT, int BF = 0>
 __global__ __launch_bounds__(kMaxBlock) __attribute__((amdgpu_waves_per_eu(4))) void k_volume_w4(
     int E, const double* __restrict__ Ue, const R* __restrict__ Us, const R* __restrict__ dUce, const R* __restrict__ JGe,
     const R* __restrict__ iJe, R* __restrict__ divFe, R* __restrict__ disFne, VFace<R> vf, Tabs tb,
     Phys ph) {
-  volume_body<D, P, VISC, R, NT>(E, Ue, Us, dUce, JGe, iJe, divFe, disFne, vf, tb, ph);
+  volume_body<D, P, VISC, R, NT, BF>(E, Ue, Us, dUce, JGe, iJe, divFe, disFne, vf, tb, ph);
 }
 
 // ---------------------------------------------------------------------------
@@ -1456,16 +1543,26 @@ struct Runner {
     FusedState& st = fs(c);
     Work<R>& w = st.work<R>();
     const VFace<R> vf{st.fpart, w.nf, w.Bve, w.gBnd, w.dA, st.nBnd};
-    // FP64 hexes need more than 128 VGPRs (P3 spills 19 under the hold): no
-    // hold there; unheld they run at 3 waves/SIMD without spills, measured
-    // 2% faster at P3 once the streaming arrays bypass the caches
+    const Tabs& tb = st.tb;
+    // quads and FP32 are held to 4 waves per SIMD (128 VGPRs); FP64 hexes,
+    // which spill under that hold, to 3 (168 VGPRs: P3 without spills, where
+    // the unheld compiler would take 211 and leave 2 waves)
+    if constexpr (kBf16<D, P, R>) {
+      if (c.bf16 == 2 || c.bf16 == 3) {
+        auto kern = c.bf16 == 2 ? k_volume_w4<D, P, VISC, R, NT, 2> : k_volume_w4<D, P, VISC, R, NT, 3>;
+        hipLaunchKernelGGL(kern, dim3(nblk(c.z.E, nt / S)), dim3(nt), lds(kVolume, nt, VISC), c.stream, c.z.E, st.Ue,
+                           w.Us, w.dUce, w.JG, w.iJ, w.divFe, w.disFne, vf, tb, c.ph);
+        FCHK(hipGetLastError());
+        return;
+      }
+    }
     if constexpr (kHold<D, R>)
       hipLaunchKernelGGL((k_volume_w4<D, P, VISC, R, NT>), dim3(nblk(c.z.E, nt / S)), dim3(nt),
                          lds(kVolume, nt, VISC), c.stream, c.z.E, st.Ue, w.Us, w.dUce, w.JG, w.iJ, w.divFe, w.disFne,
-                         vf, st.tb, c.ph);
+                         vf, tb, c.ph);
     else
       hipLaunchKernelGGL((k_volume_w3<D, P, VISC, R, NT>), dim3(nblk(c.z.E, nt / S)), dim3(nt), lds(kVolume, nt, VISC),
-                         c.stream, c.z.E, st.Ue, w.Us, w.dUce, w.JG, w.iJ, w.divFe, w.disFne, vf, st.tb, c.ph);
+                         c.stream, c.z.E, st.Ue, w.Us, w.dUce, w.JG, w.iJ, w.divFe, w.disFne, vf, tb, c.ph);
     FCHK(hipGetLastError());
   }
   static void launch_volume(Ctx& c, int nt) {

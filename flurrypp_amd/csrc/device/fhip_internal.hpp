@@ -84,10 +84,13 @@ struct Ctx {
   bool timing = false;
   Timer timer;
   hipEvent_t ev0 = nullptr, ev1 = nullptr;
-  // captured RK steps (fused path): [with P-multigrid source][FP64, mixed].
-  // A graph holds the device pointers of the moment it was captured: drop
-  // them whenever an array they use is (re)allocated.
-  hipGraphExec_t graph[2][3] = {{nullptr, nullptr, nullptr}, {nullptr, nullptr, nullptr}};
+  // captured RK steps (fused path): [with P-multigrid source][FP64, mixed,
+  // mixed with BF16 hi+lo, mixed with BF16 hi+mid+lo].  A graph holds the
+  // device pointers of the moment it was captured: drop them whenever an
+  // array they use is (re)allocated.
+  hipGraphExec_t graph[2][4] = {{nullptr, nullptr, nullptr, nullptr}, {nullptr, nullptr, nullptr, nullptr}};
+  // mixed path: BF16 pieces of the matrix-core z contractions (0: none)
+  int bf16 = 0;
   unsigned long long graph_gen = 0;  // bumped by drop_graphs: graphs over several contexts check it
   void drop_graphs() {
     graph_gen++;

@@ -140,9 +140,14 @@ enum {
   FHIP_PATH_AUTO = 0,      /* fused kernels where available, else reference-structured */
   FHIP_PATH_STAGED = 1,    /* one kernel per reference step, dense operators */
   FHIP_PATH_FUSED = 2,     /* tensor-product (sum-factorised) fused kernels, FP64 */
-  FHIP_PATH_MIXED = 3      /* fused kernels with FP32 fluxes and face/volume
+  FHIP_PATH_MIXED = 3,     /* fused kernels with FP32 fluxes and face/volume
                               intermediates; FP64 state, RK accumulation,
                               metric terms, wave speeds, dt and residual */
+  FHIP_PATH_MIXED_BF16 = 4,   /* FHIP_PATH_MIXED with the z contractions of the
+                                 operators (gradient, divergence) of P3 hexes on
+                                 the matrix cores as BF16 hi+lo products with FP32
+                                 accumulation (P3 hexes only) */
+  FHIP_PATH_MIXED_BF16X3 = 5  /* the same with three BF16 pieces (hi+mid+lo) */
 };
 
 typedef struct fhip_ctx fhip_ctx;
@@ -237,8 +242,10 @@ int fhip_update_many(fhip_ctx** ctxs, int n_ctx, int n_steps, int path);
  * smooth_steps: input key smoothSteps.  The coarse levels keep their own
  * src_spts/sol_spts/corr_spts and never refresh their wave speeds in calcDt
  * (their iteration counter stays at initIter, src/flurry.cpp:131-136).
- * fhip_pmg_cycle queues one cycle after the fine level's update; every level
- * must be driven through the same path.  The contexts must outlive mg. */
+ * fhip_pmg_cycle queues one cycle after the fine level's update, all of it on
+ * the fine level's stream: fhip_sync the fine level before using a coarse
+ * level directly.  Every level must be driven through the same path.  The
+ * contexts must outlive mg. */
 typedef struct fhip_pmg fhip_pmg;
 int fhip_pmg_create(fhip_ctx* const* levels, int n_levels, const double* const* opp_restrict,
                     const double* const* opp_prolong, int smooth_steps, fhip_pmg** out);

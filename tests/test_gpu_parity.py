@@ -298,3 +298,39 @@ def test_vortex2d_full_size_100_steps(path):
     assert err < TOL_TRAJ  # a state this close after 100 steps also means the same dt at every step
     np.testing.assert_allclose(s.residual_norm(2), o.residual_norm(2), rtol=1e-10, atol=0)
     s.close()
+
+
+# the BF16 matrix-core forms of the mixed path (z contractions of P3 hexes as
+# products of BF16 pieces, FP32 accumulation) against the FP64 oracle after
+# 10 RK steps; hi+lo keeps about 16 bits of each factor, hi+mid+lo 24
+BF16_TOL = {4: 2e-5, 5: 1e-6}
+
+
+@pytest.mark.parametrize("path", [4, 5])
+@pytest.mark.parametrize("name", ["tgv3d_p3_ns_jitter", "vortex3d_p3"])
+def test_mixed_bf16_error(name, path):
+    if name == "vortex3d_p3":
+        setup, U = cases.vortex3d(n=4, order=3)
+    else:
+        setup, U = CASES[name]()
+    p = setup.params
+    s = Solver(p, setup=setup)
+    o = Oracle.from_setup(setup)
+    s.initializeSolution(U)
+    o.initializeSolution(U, p.dtType)
+    s.update(10, path=path)
+    o.update(10)
+    a, b = s.get_U(), o.get_U()
+    err, err_rho = rel_l2(a, b), rel_l2(a[..., 0], b[..., 0])
+    print(f"{name} BF16 path {path}: U rel_l2={err:.3e} rho rel_l2={err_rho:.3e}", flush=True)
+    assert err < BF16_TOL[path] and err_rho < BF16_TOL[path]
+
+
+def test_bf16_paths_refuse_other_shapes():
+    from flurrypp_amd._native import FhipError
+    setup, U = cases.vortex2d(n=6, order=3)
+    s = Solver(setup.params, setup=setup)
+    s.initializeSolution(U)
+    with pytest.raises(FhipError, match="P3 hexes"):
+        s.update(1, path=4)
+    s.close()
