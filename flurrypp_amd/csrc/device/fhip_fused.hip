@@ -337,11 +337,21 @@ __global__ __launch_bounds__(kMaxBlock) void k_extrap(int E, const double* __res
   R* const sU = reinterpret_cast<R*>(fhip_lds);
   const int e0 = blockIdx.x * EB, nb = min(EB, E - e0);
   const double* src = Ue + (long long)e0 * N * S;
-  for (int i = threadIdx.x; i < nb * N * S; i += blockDim.x) {
-    const int r = i / S, q = i - r * S;
-    const R v = R(src[i]);
-    sU[r * S + spt_lds(D, P1, q)] = v;
-    if (Us) Us[(long long)e0 * N * S + i] = v;
+  // all loads first (at most N per thread), then the stores
+  R v[N];
+#pragma unroll
+  for (int u = 0; u < N; u++) {
+    const int i = threadIdx.x + u * blockDim.x;
+    if (i < nb * N * S) v[u] = R(src[i]);
+  }
+#pragma unroll
+  for (int u = 0; u < N; u++) {
+    const int i = threadIdx.x + u * blockDim.x;
+    if (i < nb * N * S) {
+      const int r = i / S, q = i - r * S;
+      sU[r * S + spt_lds(D, P1, q)] = v[u];
+      if (Us) Us[(long long)e0 * N * S + i] = v[u];
+    }
   }
   __syncthreads();
   for (int i = threadIdx.x; i < nb * F; i += blockDim.x) {
@@ -530,23 +540,39 @@ __device__ __forceinline__ void volume_body(int E, const double* __restrict__ Ue
     if constexpr (VISC) inv = iJe[e * S + s];
   }
   {
-    if (Us) {
-      const R* src = Us + (long long)e0 * N * S;
-      for (int i = threadIdx.x; i < nb * N * S; i += blockDim.x) {
-        const int r = i / S, q = i - r * S;
-        sU[r * S + spt_lds(D, P1, q)] = ld_s<NT>(&src[i]);
-      }
-    } else {
-      const double* src = Ue + (long long)e0 * N * S;
-      for (int i = threadIdx.x; i < nb * N * S; i += blockDim.x) {
-        const int r = i / S, q = i - r * S;
-        sU[r * S + spt_lds(D, P1, q)] = R(ld_s<NT>(&src[i]));
-      }
+    // every load of the slabs is issued before the first LDS store, so that
+    // their latencies overlap: a thread stages at most N values of U (the
+    // block holds at most blockDim / S elements) and KC of dUc
+    constexpr int KC = VISC ? (N * F + S - 1) / S : 1;
+    const int nU = nb * N * S, nC = nb * N * F;
+    R vu[N], vc[KC];
+#pragma unroll
+    for (int u = 0; u < N; u++) {
+      const int i = threadIdx.x + u * blockDim.x;
+      if (i < nU) vu[u] = Us ? ld_s<NT>(&Us[(long long)e0 * N * S + i]) : R(ld_s<NT>(&Ue[(long long)e0 * N * S + i]));
     }
     if constexpr (VISC) {
       // dUc at the block's flux points (sF serves as sC until the gradient is written)
-      const long long o = (long long)e0 * N * F;
-      for (int i = threadIdx.x; i < nb * N * F; i += blockDim.x) sF[i] = ld_s<NT>(&dUce[o + i]);
+#pragma unroll
+      for (int u = 0; u < KC; u++) {
+        const int i = threadIdx.x + u * blockDim.x;
+        if (i < nC) vc[u] = ld_s<NT>(&dUce[(long long)e0 * N * F + i]);
+      }
+    }
+#pragma unroll
+    for (int u = 0; u < N; u++) {
+      const int i = threadIdx.x + u * blockDim.x;
+      if (i < nU) {
+        const int r = i / S, q = i - r * S;
+        sU[r * S + spt_lds(D, P1, q)] = vu[u];
+      }
+    }
+    if constexpr (VISC) {
+#pragma unroll
+      for (int u = 0; u < KC; u++) {
+        const int i = threadIdx.x + u * blockDim.x;
+        if (i < nC) sF[i] = vc[u];
+      }
     }
   }
   __syncthreads();
@@ -869,36 +895,28 @@ __global__ __launch_bounds__(kMaxBlock) void k_rk(int E, int mode, double a, dou
   R* const sD = sU + EB * N * S;                  // [EB*N*F] Fn - disFn at the block's flux points
   const int e0 = blockIdx.x * EB, nb = min(EB, E - e0);
   const int b = threadIdx.x / S, s = threadIdx.x - b * S;
+  // The flux points' loads are unconditional (a boundary point reads its own
+  // row where an interior point reads its partner's, and selects) and, for
+  // the points a thread stages, issued before their LDS stores, so that their
+  // latencies overlap.  A thread stages at most KF = ceil(F / S) points.
+  constexpr int KF = (F + S - 1) / S;
   if (Bve && kJumpInVolume<R>) {
     // viscous: disFne holds this side's jump part +-B dA - disFn; add the
     // inviscid part (Fne, already times dA) and the partner side's B times dA,
     // negated on the right side
-    for (int i = threadIdx.x; i < nb * F; i += blockDim.x) {
-      const int bb = i / F, f = i - bb * F;
-      const long long ef = (long long)(e0 + bb) * F + f, ob = (long long)(e0 + bb) * N * F + f;
-      const int code = fpart[ef], side = code & 3;
-      if (side != 2) {
-        const R dA = dAe[ef];
-        const long long p = code >> 2, ep = p / F, pb = ep * N * F + (p - ep * F);
+    int cd[KF];
 #pragma unroll
-        for (int k = 0; k < N; k++) {
-          const R bp = Bve[pb + k * F];
-          sD[(bb * N + k) * F + f] = (ld_s<NT>(&Fne[ob + k * F]) +
-                                      ld_s<NT>(&disFne[ob + k * F])) + (side == 0 ? bp : -bp) * dA;
-        }
-      } else {
-#pragma unroll
-        for (int k = 0; k < N; k++)
-          sD[(bb * N + k) * F + f] = ld_s<NT>(&Fne[ob + k * F]) + ld_s<NT>(&disFne[ob + k * F]);
-      }
+    for (int u = 0; u < KF; u++) {
+      const int i = threadIdx.x + u * blockDim.x;
+      cd[u] = (i < nb * F) ? fpart[(long long)e0 * F + i] : 2;
     }
-  } else if (Bve) {
-    // viscous: common flux = inviscid part (Fne, already times dA) plus both
-    // sides' LDG parts times dA, negated on the right side
-    for (int i = threadIdx.x; i < nb * F; i += blockDim.x) {
+#pragma unroll
+    for (int u = 0; u < KF; u++) {
+      const int i = threadIdx.x + u * blockDim.x;
+      if (i >= nb * F) break;
       const int bb = i / F, f = i - bb * F;
       const long long ef = (long long)(e0 + bb) * F + f, ob = (long long)(e0 + bb) * N * F + f;
-      const int code = fpart[ef], side = code & 3;
+      const int code = cd[u], side = code & 3;
       const R dA = dAe[ef];
       long long pb = ob;
       if (side != 2) {
@@ -907,17 +925,59 @@ __global__ __launch_bounds__(kMaxBlock) void k_rk(int E, int mode, double a, dou
       }
 #pragma unroll
       for (int k = 0; k < N; k++) {
-        R bv = Bve[ob + k * F];
-        if (side != 2) bv += Bve[pb + k * F];
+        const R bp = Bve[pb + k * F];
+        const R base = ld_s<NT>(&Fne[ob + k * F]) + ld_s<NT>(&disFne[ob + k * F]);
+        sD[(bb * N + k) * F + f] = (side == 2) ? base : base + (side == 0 ? bp : -bp) * dA;
+      }
+    }
+  } else if (Bve) {
+    // viscous: common flux = inviscid part (Fne, already times dA) plus both
+    // sides' LDG parts times dA, negated on the right side
+    int cd[KF];
+#pragma unroll
+    for (int u = 0; u < KF; u++) {
+      const int i = threadIdx.x + u * blockDim.x;
+      cd[u] = (i < nb * F) ? fpart[(long long)e0 * F + i] : 2;
+    }
+#pragma unroll
+    for (int u = 0; u < KF; u++) {
+      const int i = threadIdx.x + u * blockDim.x;
+      if (i >= nb * F) break;
+      const int bb = i / F, f = i - bb * F;
+      const long long ef = (long long)(e0 + bb) * F + f, ob = (long long)(e0 + bb) * N * F + f;
+      const int code = cd[u], side = code & 3;
+      const R dA = dAe[ef];
+      long long pb = ob;
+      if (side != 2) {
+        const long long p = code >> 2, ep = p / F;
+        pb = ep * N * F + (p - ep * F);
+      }
+#pragma unroll
+      for (int k = 0; k < N; k++) {
+        const R bo = Bve[ob + k * F], bp = Bve[pb + k * F];
+        R bv = (side != 2) ? bo + bp : bo;
         if (side == 1) bv = -bv;
         sD[(bb * N + k) * F + f] = (ld_s<NT>(&Fne[ob + k * F]) + bv * dA) -
                                    ld_s<NT>(&disFne[ob + k * F]);
       }
     }
   } else {
+    constexpr int KI = (N * F + S - 1) / S;
     const long long o = (long long)e0 * N * F;
-    for (int i = threadIdx.x; i < nb * N * F; i += blockDim.x)
-      sD[i] = ld_s<NT>(&Fne[o + i]) - ld_s<NT>(&disFne[o + i]);
+    R fv[KI], dv[KI];
+#pragma unroll
+    for (int u = 0; u < KI; u++) {
+      const int i = threadIdx.x + u * blockDim.x;
+      if (i < nb * N * F) {
+        fv[u] = ld_s<NT>(&Fne[o + i]);
+        dv[u] = ld_s<NT>(&disFne[o + i]);
+      }
+    }
+#pragma unroll
+    for (int u = 0; u < KI; u++) {
+      const int i = threadIdx.x + u * blockDim.x;
+      if (i < nb * N * F) sD[i] = fv[u] - dv[u];
+    }
   }
   __syncthreads();
   if (mode == 5) {
