@@ -506,10 +506,15 @@ __device__ __forceinline__ void volume_body(int E, const double* __restrict__ Ue
   const int EB = blockDim.x / S;
   // sF holds, in turn (viscous runs): dUc at the flux points, the physical
   // gradient until it has been extrapolated to the faces, then the transformed
-  // flux at the same indices as the gradient.  [EB*N*S] + [EB*N*SF]
+  // flux at the same indices as the gradient.  [EB*N*S] + [EB*N*SF]; viscous
+  // runs also stage the faces' normals and the points' side codes for the
+  // face loop, [EB*D*F] + [EB*F] ints
+  constexpr int SF = (VISC && F > D * S) ? F : D * S;
   R* const sU = reinterpret_cast<R*>(fhip_lds);
   R* const sF = sU + EB * N * S;
   R* const sG = sF;
+  R* const sN = sF + EB * N * SF;
+  int* const sP = reinterpret_cast<int*>(sN + EB * D * F);
   const int e0 = blockIdx.x * EB, nb = min(EB, E - e0);
   const int b = threadIdx.x / S, s = threadIdx.x - b * S;
   const int sw = spt_lds(D, P1, s);
@@ -543,9 +548,11 @@ __device__ __forceinline__ void volume_body(int E, const double* __restrict__ Ue
     // every load of the slabs is issued before the first LDS store, so that
     // their latencies overlap: a thread stages at most N values of U (the
     // block holds at most blockDim / S elements) and KC of dUc
-    constexpr int KC = VISC ? (N * F + S - 1) / S : 1;
-    const int nU = nb * N * S, nC = nb * N * F;
-    R vu[N], vc[KC];
+    constexpr int KC = VISC ? (N * F + S - 1) / S : 1, KN = VISC ? (D * F + S - 1) / S : 1,
+                  KP = VISC ? (F + S - 1) / S : 1;
+    const int nU = nb * N * S, nC = nb * N * F, nN = nb * D * F, nP = nb * F;
+    R vu[N], vc[KC], vn[KN];
+    int vp[KP];
 #pragma unroll
     for (int u = 0; u < N; u++) {
       const int i = threadIdx.x + u * blockDim.x;
@@ -557,6 +564,16 @@ __device__ __forceinline__ void volume_body(int E, const double* __restrict__ Ue
       for (int u = 0; u < KC; u++) {
         const int i = threadIdx.x + u * blockDim.x;
         if (i < nC) vc[u] = ld_s<NT>(&dUce[(long long)e0 * N * F + i]);
+      }
+#pragma unroll
+      for (int u = 0; u < KN; u++) {
+        const int i = threadIdx.x + u * blockDim.x;
+        if (i < nN) vn[u] = ld_s<NT>(&vf.nfe[(long long)e0 * D * F + i]);
+      }
+#pragma unroll
+      for (int u = 0; u < KP; u++) {
+        const int i = threadIdx.x + u * blockDim.x;
+        if (i < nP) vp[u] = vf.fpart[(long long)e0 * F + i];
       }
     }
 #pragma unroll
@@ -572,6 +589,16 @@ __device__ __forceinline__ void volume_body(int E, const double* __restrict__ Ue
       for (int u = 0; u < KC; u++) {
         const int i = threadIdx.x + u * blockDim.x;
         if (i < nC) sF[i] = vc[u];
+      }
+#pragma unroll
+      for (int u = 0; u < KN; u++) {
+        const int i = threadIdx.x + u * blockDim.x;
+        if (i < nN) sN[i] = vn[u];
+      }
+#pragma unroll
+      for (int u = 0; u < KP; u++) {
+        const int i = threadIdx.x + u * blockDim.x;
+        if (i < nP) sP[i] = vp[u];
       }
     }
   }
@@ -644,11 +671,11 @@ __device__ __forceinline__ void volume_body(int E, const double* __restrict__ Ue
       const long long ee = e0 + bb;
       const int* xi = tb.ext_idx + f * P1;
       const R* xw = tw<R>(tb.ext_w, tb.ext_wf) + f * P1;
-      // side code and normal first: their latency overlaps the sums below
-      const int code = vf.fpart[ee * F + f], side = code & 3;
+      // side code and normal, staged in LDS with the slab
+      const int code = sP[bb * F + f], side = code & 3;
       R nv[D];
 #pragma unroll
-      for (int x = 0; x < D; x++) nv[x] = ld_s<NT>(&vf.nfe[(ee * D + x) * F + f]);
+      for (int x = 0; x < D; x++) nv[x] = sN[(bb * D + x) * F + f];
       R g[D][N];
 #pragma unroll
       for (int x = 0; x < D; x++)
@@ -775,7 +802,7 @@ __device__ __forceinline__ void volume_body(int E, const double* __restrict__ Ue
     int side = 2;
     R dA = 0, bv[N];
     if constexpr (VISC && kJumpInVolume<R>) {
-      side = vf.fpart[ee * F + f] & 3;
+      side = sP[bb * F + f] & 3;
       dA = vf.dA[ee * F + f];
       // loaded for every point (all in flight at once); unused at boundary points
 #pragma unroll
@@ -1569,7 +1596,7 @@ struct Runner {
     const size_t EB = nt / S;
     const size_t SF = (visc && F > D * S) ? F : D * S;
     if (k == kExtrap) return EB * N * S * sizeof(R);
-    if (k == kVolume) return EB * N * (S + SF) * sizeof(R);
+    if (k == kVolume) return EB * N * (S + SF) * sizeof(R) + (visc ? EB * (D * F * sizeof(R) + F * sizeof(int)) : 0);
     if (k == kRk) return EB * N * (S + F) * sizeof(R);
     return 0;
   }
