@@ -258,6 +258,7 @@ struct FusedState {
   int nMesh = 1;
   std::vector<int> h_meshStart;
   double *bst = nullptr, *meshDt = nullptr, *meshRes = nullptr;
+  unsigned* dtTicket = nullptr;  // blocks of the dt kernel that have finished (zero between launches)
   int *eleMesh = nullptr, *meshStart = nullptr, *active = nullptr;
   // P-multigrid source term (element-major copy of src_spts), used while use_src
   double* srce = nullptr;
@@ -274,7 +275,8 @@ struct FusedState {
     w64.release();
     w32.release();
     void* ps[] = {tab_mem, Ue,  U0e,     Uacc,    divF0e,    JGe,    detJe,  iJe,   norme, dAe, waveSpe, pL,
-                  pR,      bc,  fpart,   bndT,    nfe,       bst,    meshDt, meshRes, eleMesh, meshStart, active, srce};
+                  pR,      bc,  fpart,   bndT,    nfe,       bst,    meshDt, meshRes, eleMesh, meshStart, active, srce,
+                  dtTicket};
     for (void* p : ps)
       if (p) (void)hipFree(p);
   }
@@ -466,6 +468,173 @@ __global__ __launch_bounds__(kMaxBlock) void k_face_inv(int nFF, const int* __re
   }
 }
 
+// ---------------------------------------------------------------------------
+// Correction of the divergence (correctDivFlux), RK update (timeStepA/B
+// folded into one pass per stage, FP64) and U at the flux points for the
+// next stage.  mode: 0 first classical stage, 1 middle classical stage,
+// 2 last classical stage, 3 single-stage forward Euler, 4 Jameson-form stage.
+// Classical stages combine into U0 - sum_j b_j r_j / detJ dt in timeStepB's
+// order (src/solver.cpp:504-536): either through the FP64 accumulator Uacc, or
+// (mixed path, Rst given) from the stored stage residuals, which FP32 holds
+// exactly there since they are sums of FP32 values; both give the same bits.
+struct RkB {
+  double b[8];
+};
+// The RK update's inputs and outputs besides the flux jump and divF: the
+// stage's mode and coefficients, dt (0 global, 1 per element, 2 per mesh),
+// the FP64 state and its stage copies, batching and multigrid arrays.
+template <class R>
+struct RkArgs {
+  int mode;
+  double a, bcoef;
+  const double* detJe;
+  const double* dtp;
+  const double* eleDt;
+  int local;
+  double* Ue;
+  double* U0e;
+  double* Uacc;
+  double* divF0e;
+  const int* eleMesh;
+  const double* meshDt;
+  const int* active;
+  const double* srce;
+  R* Rst;
+  int stage;
+  RkB bs;
+  R* Us;
+  int fin;
+  R* Ufe;
+};
+
+// correctDivFlux, timeStepA/B and U at the flux points for the next stage, for
+// the block's elements: dv holds this thread's divF (one value per field),
+// sD the flux jump at the block's flux points, and sU receives the new state
+// (its previous contents are no longer read).  Ends with every thread of the
+// block past a barrier unless mode is 5 (residual only), which returns at
+// once for the whole block.
+// The arrays come in as __restrict__ parameters (from RkArgs at the call
+// below), so that the compiler may keep their loads ahead of the stores.
+template <int D, int P, class R, bool NT>
+__device__ __forceinline__ void rk_tail_p(int E, const R* dv, R* sU, const R* sD, int mode, double a, double bcoef,
+                                          const double* __restrict__ detJe, const double* __restrict__ dtp,
+                                          const double* __restrict__ eleDt, int local, double* __restrict__ Ue,
+                                          double* __restrict__ U0e, double* __restrict__ Uacc,
+                                          double* __restrict__ divF0e, const int* __restrict__ eleMesh,
+                                          const double* __restrict__ meshDt, const int* __restrict__ active,
+                                          const double* __restrict__ srce, R* __restrict__ Rst, int stage,
+                                          const RkB& bs, R* __restrict__ Us, int fin, R* __restrict__ Ufe,
+                                          const Tabs& tb) {
+  using Sh = Shape<D, P>;
+  constexpr int S = Sh::S, F = Sh::F, N = Sh::N, P1 = Sh::P1, NC = Sh::NC;
+  const int EB = blockDim.x / S;
+  const long long nS = (long long)E * N * S;
+  const int e0 = blockIdx.x * EB, nb = min(EB, E - e0);
+  const int b = threadIdx.x / S, s = threadIdx.x - b * S;
+  if (mode == 5) {
+    // residual only (calcResidual(0) + correctDivFlux): divF_spts[0], no update
+    if (b < nb) {
+      const long long e = e0 + b;
+#pragma unroll
+      for (int k = 0; k < N; k++) {
+        R c = 0;
+#pragma unroll
+        for (int j = 0; j < NC; j++) c += tw<R>(tb.cor_w, tb.cor_wf)[s * NC + j] * sD[(b * N + k) * F + tb.cor_idx[s * NC + j]];
+        st_s<NT>(double(c + dv[k]), &divF0e[(e * N + k) * S + s]);
+      }
+    }
+    return;  // uniform over the block: no barrier follows for anyone
+  }
+  if (b < nb) {
+    const long long e = e0 + b;
+    const double dt = (local == 1) ? eleDt[e] : (local == 2 ? meshDt[eleMesh[e]] : *dtp);
+    const bool frozen = active && !active[eleMesh[e]];
+    const double dj = ld_s<NT>(&detJe[e * S + s]);
+    // the stage's state inputs for every field first (uniform branches on the
+    // mode), so that their loads are in flight together: U itself is an input
+    // of the first and the single stage only (and of frozen batch members,
+    // which keep it)
+    double uin[N], u0in[N], accin[N];
+    const bool needU = mode == 0 || mode == 3 || frozen;
+    const bool needU0 = mode == 1 || mode == 4 || (mode == 2 && Rst);
+    const bool needAcc = !Rst && (mode == 1 || mode == 2);
+#pragma unroll
+    for (int k = 0; k < N; k++) uin[k] = u0in[k] = accin[k] = 0.0;
+    if (needU) {
+#pragma unroll
+      for (int k = 0; k < N; k++) uin[k] = ld_s<NT>(&Ue[(e * N + k) * S + s]);
+    }
+    if (needU0) {
+#pragma unroll
+      for (int k = 0; k < N; k++) u0in[k] = ld_s<NT>(&U0e[(e * N + k) * S + s]);
+    }
+    if (needAcc) {
+#pragma unroll
+      for (int k = 0; k < N; k++) accin[k] = ld_s<NT>(&Uacc[(e * N + k) * S + s]);
+    }
+#pragma unroll
+    for (int k = 0; k < N; k++) {
+      const long long iu = (e * N + k) * S + s;
+      R c = 0;
+#pragma unroll
+      for (int j = 0; j < NC; j++) c += tw<R>(tb.cor_w, tb.cor_wf)[s * NC + j] * sD[(b * N + k) * F + tb.cor_idx[s * NC + j]];
+      const double r0 = double(c + dv[k]);
+      if (divF0e) st_s<NT>(r0, &divF0e[iu]);
+      // P-multigrid source: divF_spts[step] + src_spts (src/solver.cpp:480,514)
+      const double r = srce ? r0 + srce[iu] : r0;
+      const double u = uin[k];
+      double un;
+      if (mode == 0) {
+        st_s<NT>(u, &U0e[iu]);
+        if (Rst) st_s<NT>(R(r), &Rst[iu]);
+        else st_s<NT>(u - bcoef * r / dj * dt, &Uacc[iu]);
+        un = u - a * r / dj * dt;
+      } else if (mode == 1) {
+        if (Rst) st_s<NT>(R(r), &Rst[stage * nS + iu]);
+        else st_s<NT>(accin[k] - bcoef * r / dj * dt, &Uacc[iu]);
+        un = u0in[k] - a * r / dj * dt;
+      } else if (mode == 2) {
+        double acc;
+        if (Rst) {
+          acc = u0in[k];
+          for (int j = 0; j < stage; j++) acc = acc - bs.b[j] * double(ld_s<NT>(&Rst[j * nS + iu])) / dj * dt;
+        } else {
+          acc = accin[k];
+        }
+        un = acc - bcoef * r / dj * dt;
+      } else if (mode == 3) {
+        un = u - bcoef * r / dj * dt;
+      } else {
+        un = u0in[k] - a * r / dj * dt;
+      }
+      if (frozen) un = u;
+      // with an FP32 stage copy, the FP64 state is written at the step's
+      // last stage only (earlier stages leave it at the step's start)
+      if (!Us || fin) st_s<NT>(un, &Ue[iu]);
+      if (Us) st_s<NT>(R(un), &Us[iu]);
+      sU[(b * N + k) * S + spt_lds(D, P1, s)] = R(un);
+    }
+  }
+  __syncthreads();
+  for (int i = threadIdx.x; i < nb * F; i += blockDim.x) {
+    int bb = i / F, f = i - bb * F;
+#pragma unroll
+    for (int k = 0; k < N; k++) {
+      R acc = 0;
+#pragma unroll
+      for (int m = 0; m < P1; m++) acc += tw<R>(tb.ext_w, tb.ext_wf)[f * P1 + m] * sU[(bb * N + k) * S + tb.ext_idx[f * P1 + m]];
+      st_s<NT>(acc, &Ufe[((long long)(e0 + bb) * N + k) * F + f]);
+    }
+  }
+}
+
+template <int D, int P, class R, bool NT>
+__device__ __forceinline__ void rk_tail(int E, const R* dv, R* sU, const R* sD, const RkArgs<R>& ra, const Tabs& tb) {
+  rk_tail_p<D, P, R, NT>(E, dv, sU, sD, ra.mode, ra.a, ra.bcoef, ra.detJe, ra.dtp, ra.eleDt, ra.local, ra.Ue, ra.U0e,
+                         ra.Uacc, ra.divF0e, ra.eleMesh, ra.meshDt, ra.active, ra.srce, ra.Rst, ra.stage, ra.bs, ra.Us,
+                         ra.fin, ra.Ufe, tb);
+}
+
 // Viscous runs, FP32: the volume kernel also folds its own side's LDG part
 // into the flux jump (disFne := +-B dA - disFn), so the RK kernel reads one
 // array less.  FP64 keeps disFn and B apart: its volume kernel is short of
@@ -500,21 +669,23 @@ __device__ __forceinline__ void volume_body(int E, const double* __restrict__ Ue
                                             const R* __restrict__ dUce,
                                             const R* __restrict__ JGe, const R* __restrict__ iJe,
                                             R* __restrict__ divFe, R* __restrict__ disFne, VFace<R> vf,
-                                            Tabs tb, Phys ph) {
+                                            Tabs tb, Phys ph, const R* __restrict__ Fne, const RkArgs<R>& ra) {
   using Sh = Shape<D, P>;
-  constexpr int S = Sh::S, F = Sh::F, N = Sh::N, P1 = Sh::P1, NC = Sh::NC;
+  constexpr int S = Sh::S, F = Sh::F, N = Sh::N, P1 = Sh::P1;
   const int EB = blockDim.x / S;
   // sF holds, in turn (viscous runs): dUc at the flux points, the physical
   // gradient until it has been extrapolated to the faces, then the transformed
   // flux at the same indices as the gradient.  [EB*N*S] + [EB*N*SF]; viscous
   // runs also stage the faces' normals and the points' side codes for the
-  // face loop, [EB*D*F] + [EB*F] ints
+  // face loop, [EB*D*F] + [EB*F] ints, inviscid runs keep the flux jump of
+  // the RK update there, [EB*N*F]
   constexpr int SF = (VISC && F > D * S) ? F : D * S;
   R* const sU = reinterpret_cast<R*>(fhip_lds);
   R* const sF = sU + EB * N * S;
   R* const sG = sF;
   R* const sN = sF + EB * N * SF;
   int* const sP = reinterpret_cast<int*>(sN + EB * D * F);
+  R* const sD = sN;
   const int e0 = blockIdx.x * EB, nb = min(EB, E - e0);
   const int b = threadIdx.x / S, s = threadIdx.x - b * S;
   const int sw = spt_lds(D, P1, s);
@@ -757,6 +928,7 @@ __device__ __forceinline__ void volume_body(int E, const double* __restrict__ Ue
       for (int k = 0; k < N; k++) sF[((b * D + d1) * N + k) * S + sw] = Ft[d1][k];
   }
   __syncthreads();
+  R dv[N] = {};  // inviscid runs: divF at the point, for the RK update below
   if (act) {
     // divergence (calcDivF_spts): sum over directions in order (matrix
     // cores: y on the vector unit, then z and x added)
@@ -783,7 +955,8 @@ __device__ __forceinline__ void volume_body(int E, const double* __restrict__ Ue
         tot = mfma444(opv, sF[((b * D + 2) * N + k) * S + sw], tot);
         tot = mfma444(sF[((b * D + 0) * N + k) * S + swx], opv, tot);
       }
-      st_s<NT>(tot, &divFe[(e * N + k) * S + s]);
+      if constexpr (VISC) st_s<NT>(tot, &divFe[(e * N + k) * S + s]);
+      else dv[k] = tot;
     }
   }
   // faces: discontinuous normal flux.  With kJumpInVolume, viscous runs store
@@ -808,14 +981,26 @@ __device__ __forceinline__ void volume_body(int E, const double* __restrict__ Ue
 #pragma unroll
       for (int k = 0; k < N; k++) bv[k] = vf.Bve[(ee * N + k) * F + f];
     }
+    if constexpr (!VISC) {
+      // the common flux (times dA) the face kernel left at this point
+#pragma unroll
+      for (int k = 0; k < N; k++) bv[k] = ld_s<NT>(&Fne[(ee * N + k) * F + f]);
+    }
 #pragma unroll
     for (int k = 0; k < N; k++) {
       R a = 0;
 #pragma unroll
       for (int m = 0; m < P1; m++) a += ew[m] * sF[((bb * D + dn) * N + k) * S + xi[m]];
       if constexpr (VISC && kJumpInVolume<R>) a = (side == 2) ? -a : (side == 0 ? bv[k] : -bv[k]) * dA - a;
-      st_s<NT>(a, &disFne[(ee * N + k) * F + f]);
+      if constexpr (VISC) st_s<NT>(a, &disFne[(ee * N + k) * F + f]);
+      else sD[(bb * N + k) * F + f] = bv[k] - a;
     }
+  }
+  if constexpr (!VISC) {
+    // inviscid runs: the correction, RK update and U at the flux points here
+    // (no partner data is needed past the face kernel's common flux)
+    __syncthreads();
+    rk_tail<D, P, R, NT>(E, dv, sU, sD, ra, tb);
   }
 }
 
@@ -829,15 +1014,15 @@ template <int D, int P, bool VISC, class R, bool NT>
 __global__ __launch_bounds__(kMaxBlock) __attribute__((amdgpu_waves_per_eu(3))) void k_volume_w3(
     int E, const double* __restrict__ Ue, const R* __restrict__ Us, const R* __restrict__ dUce, const R* __restrict__ JGe,
     const R* __restrict__ iJe, R* __restrict__ divFe, R* __restrict__ disFne, VFace<R> vf, Tabs tb,
-    Phys ph) {
-  volume_body<D, P, VISC, R, NT>(E, Ue, Us, dUce, JGe, iJe, divFe, disFne, vf, tb, ph);
+    Phys ph, const R* __restrict__ Fne, RkArgs<R> ra) {
+  volume_body<D, P, VISC, R, NT>(E, Ue, Us, dUce, JGe, iJe, divFe, disFne, vf, tb, ph, Fne, ra);
 }
 template <int D, int P, bool VISC, class R, bool NT, int BF = 0>
 __global__ __launch_bounds__(kMaxBlock) __attribute__((amdgpu_waves_per_eu(4))) void k_volume_w4(
     int E, const double* __restrict__ Ue, const R* __restrict__ Us, const R* __restrict__ dUce, const R* __restrict__ JGe,
     const R* __restrict__ iJe, R* __restrict__ divFe, R* __restrict__ disFne, VFace<R> vf, Tabs tb,
-    Phys ph) {
-  volume_body<D, P, VISC, R, NT, BF>(E, Ue, Us, dUce, JGe, iJe, divFe, disFne, vf, tb, ph);
+    Phys ph, const R* __restrict__ Fne, RkArgs<R> ra) {
+  volume_body<D, P, VISC, R, NT, BF>(E, Ue, Us, dUce, JGe, iJe, divFe, disFne, vf, tb, ph, Fne, ra);
 }
 
 // ---------------------------------------------------------------------------
@@ -887,37 +1072,19 @@ __global__ __launch_bounds__(kMaxBlock) void k_face_bnd(int nB, const int* __res
   }
 }
 
-// ---------------------------------------------------------------------------
-// Correction of the divergence (correctDivFlux), RK update (timeStepA/B
-// folded into one pass per stage, FP64) and U at the flux points for the
-// next stage.  mode: 0 first classical stage, 1 middle classical stage,
-// 2 last classical stage, 3 single-stage forward Euler, 4 Jameson-form stage.
-// Classical stages combine into U0 - sum_j b_j r_j / detJ dt in timeStepB's
-// order (src/solver.cpp:504-536): either through the FP64 accumulator Uacc, or
-// (mixed path, Rst given) from the stored stage residuals, which FP32 holds
-// exactly there since they are sums of FP32 values; both give the same bits.
-struct RkB {
-  double b[8];
-};
-// With NT, the arrays this kernel reads or writes exactly once per launch go
-// through nontemporal loads and stores (they stream past the caches); arrays
-// read twice (the LDG side parts, read by both sides of a face) stay cached.
+// Correction of the divergence, RK update and U at the flux points, after the
+// viscous volume kernel.  With NT, the arrays this kernel reads or writes
+// exactly once per launch go through nontemporal loads and stores (they
+// stream past the caches); arrays read twice (the LDG side parts, read by
+// both sides of a face) stay cached.
 template <int D, int P, class R, bool NT>
-__global__ __launch_bounds__(kMaxBlock) void k_rk(int E, int mode, double a, double bcoef, const R* __restrict__ divFe,
-                                             const R* __restrict__ Fne, const R* __restrict__ disFne,
-                                             const double* __restrict__ detJe, const double* __restrict__ dtp,
-                                             const double* __restrict__ eleDt, int local, double* __restrict__ Ue,
-                                             double* __restrict__ U0e, double* __restrict__ Uacc,
-                                             double* __restrict__ divF0e, R* __restrict__ Ufe, Tabs tb,
-                                             const int* __restrict__ eleMesh, const double* __restrict__ meshDt,
-                                             const int* __restrict__ active, const double* __restrict__ srce,
-                                             R* __restrict__ Rst, int stage, RkB bs, const int* __restrict__ fpart,
-                                             const R* __restrict__ dAe, const R* __restrict__ Bve, R* __restrict__ Us,
-                                             int fin) {
+__global__ __launch_bounds__(kMaxBlock) void k_rk(int E, const R* __restrict__ divFe, const R* __restrict__ Fne,
+                                             const R* __restrict__ disFne, RkArgs<R> ra, Tabs tb,
+                                             const int* __restrict__ fpart, const R* __restrict__ dAe,
+                                             const R* __restrict__ Bve) {
   using Sh = Shape<D, P>;
-  constexpr int S = Sh::S, F = Sh::F, N = Sh::N, P1 = Sh::P1, NC = Sh::NC;
+  constexpr int S = Sh::S, F = Sh::F, N = Sh::N;
   const int EB = blockDim.x / S;
-  const long long nS = (long long)E * N * S;
   R* const sU = reinterpret_cast<R*>(fhip_lds);  // [EB*N*S]
   R* const sD = sU + EB * N * S;                  // [EB*N*F] Fn - disFn at the block's flux points
   const int e0 = blockIdx.x * EB, nb = min(EB, E - e0);
@@ -927,7 +1094,7 @@ __global__ __launch_bounds__(kMaxBlock) void k_rk(int E, int mode, double a, dou
   // the points a thread stages, issued before their LDS stores, so that their
   // latencies overlap.  A thread stages at most KF = ceil(F / S) points.
   constexpr int KF = (F + S - 1) / S;
-  if (Bve && kJumpInVolume<R>) {
+  if constexpr (kJumpInVolume<R>) {
     // viscous: disFne holds this side's jump part +-B dA - disFn; add the
     // inviscid part (Fne, already times dA) and the partner side's B times dA,
     // negated on the right side
@@ -957,7 +1124,7 @@ __global__ __launch_bounds__(kMaxBlock) void k_rk(int E, int mode, double a, dou
         sD[(bb * N + k) * F + f] = (side == 2) ? base : base + (side == 0 ? bp : -bp) * dA;
       }
     }
-  } else if (Bve) {
+  } else {
     // viscous: common flux = inviscid part (Fne, already times dA) plus both
     // sides' LDG parts times dA, negated on the right side
     int cd[KF];
@@ -988,101 +1155,14 @@ __global__ __launch_bounds__(kMaxBlock) void k_rk(int E, int mode, double a, dou
                                    ld_s<NT>(&disFne[ob + k * F]);
       }
     }
-  } else {
-    constexpr int KI = (N * F + S - 1) / S;
-    const long long o = (long long)e0 * N * F;
-    R fv[KI], dv[KI];
-#pragma unroll
-    for (int u = 0; u < KI; u++) {
-      const int i = threadIdx.x + u * blockDim.x;
-      if (i < nb * N * F) {
-        fv[u] = ld_s<NT>(&Fne[o + i]);
-        dv[u] = ld_s<NT>(&disFne[o + i]);
-      }
-    }
-#pragma unroll
-    for (int u = 0; u < KI; u++) {
-      const int i = threadIdx.x + u * blockDim.x;
-      if (i < nb * N * F) sD[i] = fv[u] - dv[u];
-    }
   }
   __syncthreads();
-  if (mode == 5) {
-    // residual only (calcResidual(0) + correctDivFlux): divF_spts[0], no update
-    if (b < nb) {
-      const long long e = e0 + b;
-#pragma unroll
-      for (int k = 0; k < N; k++) {
-        R c = 0;
-#pragma unroll
-        for (int j = 0; j < NC; j++) c += tw<R>(tb.cor_w, tb.cor_wf)[s * NC + j] * sD[(b * N + k) * F + tb.cor_idx[s * NC + j]];
-        st_s<NT>(double(c + ld_s<NT>(&divFe[(e * N + k) * S + s])), &divF0e[(e * N + k) * S + s]);
-      }
-    }
-    return;  // uniform over the block: no barrier follows for anyone
-  }
+  R dv[N];
   if (b < nb) {
-    const long long e = e0 + b;
-    // dt: 0 global, 1 per element (local CFL), 2 per mesh (batched global CFL)
-    const double dt = (local == 1) ? eleDt[e] : (local == 2 ? meshDt[eleMesh[e]] : *dtp);
-    const bool frozen = active && !active[eleMesh[e]];
-    const double dj = ld_s<NT>(&detJe[e * S + s]);
 #pragma unroll
-    for (int k = 0; k < N; k++) {
-      const long long iu = (e * N + k) * S + s;
-      R c = 0;
-#pragma unroll
-      for (int j = 0; j < NC; j++) c += tw<R>(tb.cor_w, tb.cor_wf)[s * NC + j] * sD[(b * N + k) * F + tb.cor_idx[s * NC + j]];
-      const double r0 = double(c + ld_s<NT>(&divFe[iu]));
-      if (divF0e) st_s<NT>(r0, &divF0e[iu]);
-      // P-multigrid source: divF_spts[step] + src_spts (src/solver.cpp:480,514)
-      const double r = srce ? r0 + srce[iu] : r0;
-      // U itself is an input of the first and the single stage only (and of
-      // frozen batch members, which keep it)
-      const double u = (mode == 0 || mode == 3 || frozen) ? ld_s<NT>(&Ue[iu]) : 0.0;
-      double un;
-      if (mode == 0) {
-        st_s<NT>(u, &U0e[iu]);
-        if (Rst) st_s<NT>(R(r), &Rst[iu]);
-        else st_s<NT>(u - bcoef * r / dj * dt, &Uacc[iu]);
-        un = u - a * r / dj * dt;
-      } else if (mode == 1) {
-        if (Rst) st_s<NT>(R(r), &Rst[stage * nS + iu]);
-        else st_s<NT>(ld_s<NT>(&Uacc[iu]) - bcoef * r / dj * dt, &Uacc[iu]);
-        un = ld_s<NT>(&U0e[iu]) - a * r / dj * dt;
-      } else if (mode == 2) {
-        double acc;
-        if (Rst) {
-          acc = ld_s<NT>(&U0e[iu]);
-          for (int j = 0; j < stage; j++) acc = acc - bs.b[j] * double(ld_s<NT>(&Rst[j * nS + iu])) / dj * dt;
-        } else {
-          acc = ld_s<NT>(&Uacc[iu]);
-        }
-        un = acc - bcoef * r / dj * dt;
-      } else if (mode == 3) {
-        un = u - bcoef * r / dj * dt;
-      } else {
-        un = ld_s<NT>(&U0e[iu]) - a * r / dj * dt;
-      }
-      if (frozen) un = u;
-      // with an FP32 stage copy, the FP64 state is written at the step's
-      // last stage only (earlier stages leave it at the step's start)
-      if (!Us || fin) st_s<NT>(un, &Ue[iu]);
-      if (Us) st_s<NT>(R(un), &Us[iu]);
-      sU[(b * N + k) * S + spt_lds(D, P1, s)] = R(un);
-    }
+    for (int k = 0; k < N; k++) dv[k] = ld_s<NT>(&divFe[((e0 + b) * (long long)N + k) * S + s]);
   }
-  __syncthreads();
-  for (int i = threadIdx.x; i < nb * F; i += blockDim.x) {
-    int bb = i / F, f = i - bb * F;
-#pragma unroll
-    for (int k = 0; k < N; k++) {
-      R acc = 0;
-#pragma unroll
-      for (int m = 0; m < P1; m++) acc += tw<R>(tb.ext_w, tb.ext_wf)[f * P1 + m] * sU[(bb * N + k) * S + tb.ext_idx[f * P1 + m]];
-      st_s<NT>(acc, &Ufe[((long long)(e0 + bb) * N + k) * F + f]);
-    }
-  }
+  rk_tail<D, P, R, NT>(E, dv, sU, sD, ra, tb);
 }
 
 __global__ void k_copy_em(long long n, double* __restrict__ dst, const double* __restrict__ src) {
@@ -1112,12 +1192,16 @@ __global__ void k_wave_em(int E, const R* __restrict__ Ufe, const double* __rest
   waveSpe[i] = (fabs(vN - 0.) + sqrt(csq));
 }
 
-// ele::calcDt per element (element-major inputs) and block minima.
+// ele::calcDt per element (element-major inputs) and block minima.  With a
+// ticket counter, the block that finishes last also reduces the block minima
+// to the global dt (solver::calcDt's minimum, src/solver.cpp:447-468) and
+// resets the counter to zero for the next launch.
 template <int D, int P>
 __global__ __launch_bounds__(256) void k_ele_dt_em(int E, const double* __restrict__ detJe, const double* __restrict__ ws,
                                                     const double* __restrict__ wf, const double* __restrict__ waveSpe,
                                                     const double* __restrict__ dAe, double cflfac,
-                                                    double* __restrict__ eleDt, double* __restrict__ part) {
+                                                    double* __restrict__ eleDt, double* part, unsigned* ticket,
+                                                    double* __restrict__ out) {
   using Sh = Shape<D, P>;
   constexpr int S = Sh::S, F = Sh::F, Ff = Sh::Ff;
   __shared__ double red[256];
@@ -1138,19 +1222,28 @@ __global__ __launch_bounds__(256) void k_ele_dt_em(int E, const double* __restri
     __syncthreads();
   }
   if (threadIdx.x == 0) part[blockIdx.x] = red[0];
-}
-
-__global__ void k_min_final_f(int n, const double* __restrict__ part, double* __restrict__ out) {
-  __shared__ double red[256];
+  if (!ticket) return;
+  __shared__ bool last;
+  if (threadIdx.x == 0) {
+    // the block minimum above is released with the ticket; the last taker
+    // acquires every other block's
+    const unsigned t = __hip_atomic_fetch_add(ticket, 1u, __ATOMIC_ACQ_REL, __HIP_MEMORY_SCOPE_AGENT);
+    last = t == gridDim.x - 1;
+  }
+  __syncthreads();
+  if (!last) return;
   double v = INFINITY;
-  for (int i = threadIdx.x; i < n; i += blockDim.x) v = fmin(v, part[i]);
+  for (int i = threadIdx.x; i < (int)gridDim.x; i += blockDim.x) v = fmin(v, part[i]);
   red[threadIdx.x] = v;
   __syncthreads();
   for (int st = blockDim.x / 2; st > 0; st >>= 1) {
     if (threadIdx.x < st) red[threadIdx.x] = fmin(red[threadIdx.x], red[threadIdx.x + st]);
     __syncthreads();
   }
-  if (threadIdx.x == 0) *out = red[0];
+  if (threadIdx.x == 0) {
+    *out = red[0];
+    __hip_atomic_store(ticket, 0u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+  }
 }
 
 // Per-mesh minimum of the element time steps (batched contexts, dt_type 1).
@@ -1377,11 +1470,12 @@ static void ensure_work(Ctx& c) {
   Work<R>& w = fs(c).work<R>();
   if (w.ready) return;
   const long long E = c.z.E, S = c.z.S, F = c.z.F, N = c.z.N, D = c.z.D;
-  w.divFe = dal<R>(E * N * S);
   w.Ufe = dal<R>(E * N * F);
   w.Fne = dal<R>(E * N * F);
-  w.disFne = dal<R>(E * N * F);
   if (c.prm.viscous) {
+    // inviscid runs keep divF and the discontinuous normal flux in the volume kernel
+    w.divFe = dal<R>(E * N * S);
+    w.disFne = dal<R>(E * N * F);
     w.dUce = dal<R>(E * N * F);
     w.Bve = dal<R>(E * N * F);
     w.gBnd = dal<R>((long long)D * N * fs(c).nBnd);
@@ -1416,6 +1510,8 @@ bool fused_setup(Ctx& c, const fhip_desc* d) {
   st->Uacc = dal<double>(E * N * S);
   st->divF0e = dal<double>(E * N * S);
   st->waveSpe = dal<double>(E * F);
+  st->dtTicket = dal<unsigned>(1);
+  FCHK(hipMemset(st->dtTicket, 0, sizeof(unsigned)));
   st->JGe = dal<double>(E * D * D * S);
   st->detJe = dal<double>(E * S);
   st->iJe = dal<double>(E * S);
@@ -1596,7 +1692,8 @@ struct Runner {
     const size_t EB = nt / S;
     const size_t SF = (visc && F > D * S) ? F : D * S;
     if (k == kExtrap) return EB * N * S * sizeof(R);
-    if (k == kVolume) return EB * N * (S + SF) * sizeof(R) + (visc ? EB * (D * F * sizeof(R) + F * sizeof(int)) : 0);
+    if (k == kVolume)
+      return EB * N * (S + SF) * sizeof(R) + (visc ? EB * (D * F * sizeof(R) + F * sizeof(int)) : EB * N * F * sizeof(R));
     if (k == kRk) return EB * N * (S + F) * sizeof(R);
     return 0;
   }
@@ -1626,7 +1723,7 @@ struct Runner {
     FCHK(hipGetLastError());
   }
   template <bool VISC, bool NT>
-  static void launch_volume_t(Ctx& c, int nt) {
+  static void launch_volume_t(Ctx& c, int nt, const RkArgs<R>& ra) {
     FusedState& st = fs(c);
     Work<R>& w = st.work<R>();
     const VFace<R> vf{st.fpart, w.nf, w.Bve, w.gBnd, w.dA, st.nBnd};
@@ -1638,7 +1735,7 @@ struct Runner {
       if (c.bf16 == 2 || c.bf16 == 3) {
         auto kern = c.bf16 == 2 ? k_volume_w4<D, P, VISC, R, NT, 2> : k_volume_w4<D, P, VISC, R, NT, 3>;
         hipLaunchKernelGGL(kern, dim3(nblk(c.z.E, nt / S)), dim3(nt), lds(kVolume, nt, VISC), c.stream, c.z.E, st.Ue,
-                           w.Us, w.dUce, w.JG, w.iJ, w.divFe, w.disFne, vf, tb, c.ph);
+                           w.Us, w.dUce, w.JG, w.iJ, w.divFe, w.disFne, vf, tb, c.ph, w.Fne, ra);
         FCHK(hipGetLastError());
         return;
       }
@@ -1646,16 +1743,18 @@ struct Runner {
     if constexpr (kHold<D, R>)
       hipLaunchKernelGGL((k_volume_w4<D, P, VISC, R, NT>), dim3(nblk(c.z.E, nt / S)), dim3(nt),
                          lds(kVolume, nt, VISC), c.stream, c.z.E, st.Ue, w.Us, w.dUce, w.JG, w.iJ, w.divFe, w.disFne,
-                         vf, tb, c.ph);
+                         vf, tb, c.ph, w.Fne, ra);
     else
       hipLaunchKernelGGL((k_volume_w3<D, P, VISC, R, NT>), dim3(nblk(c.z.E, nt / S)), dim3(nt), lds(kVolume, nt, VISC),
-                         c.stream, c.z.E, st.Ue, w.Us, w.dUce, w.JG, w.iJ, w.divFe, w.disFne, vf, tb, c.ph);
+                         c.stream, c.z.E, st.Ue, w.Us, w.dUce, w.JG, w.iJ, w.divFe, w.disFne, vf, tb, c.ph, w.Fne, ra);
     FCHK(hipGetLastError());
   }
-  static void launch_volume(Ctx& c, int nt) {
+  // viscous runs: the volume kernel (ra unused); inviscid runs: the volume
+  // kernel with the RK update of stage ra
+  static void launch_volume(Ctx& c, int nt, const RkArgs<R>& ra) {
     const bool sm = fs(c).stream && kEleRowsWhole;
-    if (c.prm.viscous) sm ? launch_volume_t<true, true>(c, nt) : launch_volume_t<true, false>(c, nt);
-    else sm ? launch_volume_t<false, true>(c, nt) : launch_volume_t<false, false>(c, nt);
+    if (c.prm.viscous) sm ? launch_volume_t<true, true>(c, nt, ra) : launch_volume_t<true, false>(c, nt, ra);
+    else sm ? launch_volume_t<false, true>(c, nt, ra) : launch_volume_t<false, false>(c, nt, ra);
   }
   static void launch_face_bnd(Ctx& c, int nt) {
     FusedState& st = fs(c);
@@ -1665,20 +1764,25 @@ struct Runner {
                        st.bc, w.Ufe, w.gBnd, w.nrm, w.dA, w.Bve, w.disFne, c.ph, st.bst);
     FCHK(hipGetLastError());
   }
-  static void launch_rk(Ctx& c, int nt, int mode, double a, double b, bool write0, int stage = 0) {
+  static RkArgs<R> rk_args(Ctx& c, int mode, double a, double b, bool write0, int stage) {
     FusedState& st = fs(c);
     Work<R>& w = st.work<R>();
     RkB bs{};
     for (int j = 0; j < c.z.nRK && j < 8; j++) bs.b[j] = c.prm.rk_b[j];
-    R* rst = (w.Rst && !st.use_src) ? w.Rst : nullptr;
+    return RkArgs<R>{mode, a, b, st.detJe, c.d_dt, c.eleDt,
+                     c.prm.dt_type == 2 ? 1 : (c.prm.dt_type == 1 && st.nMesh > 1 ? 2 : 0), st.Ue, st.U0e, st.Uacc,
+                     write0 ? st.divF0e : nullptr, st.eleMesh, st.meshDt, st.active, st.use_src ? st.srce : nullptr,
+                     (w.Rst && !st.use_src) ? w.Rst : nullptr, stage, bs, w.Us,
+                     (mode == 2 || mode == 3 || (mode == 4 && stage == c.z.nRK - 1)) ? 1 : 0, w.Ufe};
+  }
+  // the RK kernel of viscous runs; inviscid runs do its work at the end of
+  // their volume kernel
+  static void launch_rk(Ctx& c, int nt, int mode, double a, double b, bool write0, int stage = 0) {
+    FusedState& st = fs(c);
+    Work<R>& w = st.work<R>();
     auto* kern = (st.stream && kEleRowsWhole) ? k_rk<D, P, R, true> : k_rk<D, P, R, false>;
-    hipLaunchKernelGGL(kern, dim3(nblk(c.z.E, nt / S)), dim3(nt), lds(kRk, nt, false), c.stream, c.z.E, mode, a, b,
-                       w.divFe, w.Fne, w.disFne, st.detJe, c.d_dt, c.eleDt,
-                       c.prm.dt_type == 2 ? 1 : (c.prm.dt_type == 1 && st.nMesh > 1 ? 2 : 0), st.Ue, st.U0e, st.Uacc,
-                       write0 ? st.divF0e : nullptr, w.Ufe, st.tb, st.eleMesh, st.meshDt, st.active,
-                       st.use_src ? st.srce : nullptr, rst, stage, bs, st.fpart, w.dA,
-                       c.prm.viscous ? w.Bve : nullptr, w.Us,
-                       (mode == 2 || mode == 3 || (mode == 4 && stage == c.z.nRK - 1)) ? 1 : 0);
+    hipLaunchKernelGGL(kern, dim3(nblk(c.z.E, nt / S)), dim3(nt), lds(kRk, nt, false), c.stream, c.z.E, w.divFe, w.Fne,
+                       w.disFne, rk_args(c, mode, a, b, write0, stage), st.tb, st.fpart, w.dA, w.Bve);
     FCHK(hipGetLastError());
   }
   static void extrap(Ctx& c) {
@@ -1697,42 +1801,52 @@ struct Runner {
     FusedState& st = fs(c);
     unsigned nb = nblk(c.z.E, 256);
     {
+      // one mesh: the element kernel's last block forms the global minimum
       Launch L(c, nm("fused_ele_dt", "mixed_ele_dt"));
+      const bool one = st.nMesh == 1;
       hipLaunchKernelGGL((k_ele_dt_em<D, P>), dim3(nb), dim3(256), 0, c.stream, c.z.E, st.detJe, c.ws, c.wf, st.waveSpe,
-                         st.dAe, cflfac, c.eleDt, c.partial);
+                         st.dAe, cflfac, c.eleDt, c.partial, one ? st.dtTicket : nullptr, c.d_dt);
       FCHK(hipGetLastError());
     }
-    Launch L(c, nm("fused_dt_min", "mixed_dt_min"));
-    if (st.nMesh > 1)
+    if (st.nMesh > 1) {
+      Launch L(c, nm("fused_dt_min", "mixed_dt_min"));
       hipLaunchKernelGGL(k_mesh_dt_min, dim3(st.nMesh), dim3(256), 0, c.stream, st.meshStart, c.eleDt, st.meshDt);
-    else
-      hipLaunchKernelGGL(k_min_final_f, dim3(1), dim3(256), 0, c.stream, (int)nb, c.partial, c.d_dt);
-    FCHK(hipGetLastError());
+      FCHK(hipGetLastError());
+    }
   }
+  // calcResidual up to what the RK update needs: viscous runs the face,
+  // volume and boundary kernels; inviscid runs the face kernel only (their
+  // volume kernel runs with the update, in rk below)
   static void residual(Ctx& c) {
     const bool visc = c.prm.viscous != 0;
     {
       Launch L(c, nm("fused_face_inviscid", "mixed_face_inviscid"));
       launch_face_inv(c, blk(c, kFaceInv));
     }
+    if (!visc) return;
     {
-      Launch L(c, visc ? nm("fused_volume_viscous", "mixed_volume_viscous")
-                       : nm("fused_volume_inviscid", "mixed_volume_inviscid"));
-      launch_volume(c, blk(c, kVolume));
+      Launch L(c, nm("fused_volume_viscous", "mixed_volume_viscous"));
+      launch_volume(c, blk(c, kVolume), RkArgs<R>{});
     }
-    if (visc && fs(c).nBnd > 0) {
+    if (fs(c).nBnd > 0) {
       Launch L(c, nm("fused_face_boundary", "mixed_face_boundary"));
       launch_face_bnd(c, blk(c, kFaceBnd));
     }
   }
   static void rk(Ctx& c, int mode, double a, double b, bool write0, int stage) {
+    if (!c.prm.viscous) {
+      Launch L(c, nm("fused_volume_rk", "mixed_volume_rk"));
+      launch_volume(c, blk(c, kVolume), rk_args(c, mode, a, b, write0, stage));
+      return;
+    }
     Launch L(c, nm("fused_rk_update", "mixed_rk_update"));
     launch_rk(c, blk(c, kRk), mode, a, b, write0, stage);
   }
   // Time every valid block size of every kernel on this context's data and
-  // keep the fastest.  The RK kernel runs in its middle-stage mode, which
-  // rewrites U and the RK accumulator: both are saved and restored, as are the
-  // wave speeds the face kernel writes; block shapes never change results.
+  // keep the fastest.  The RK kernel (and the volume kernel of inviscid runs,
+  // which includes the update) runs in its middle-stage mode, which rewrites
+  // U and the RK accumulator: both are saved and restored, as are the wave
+  // speeds the face kernel writes; block shapes never change results.
   static void tune(Ctx& c) {
     FusedState& st = fs(c);
     const bool visc = c.prm.viscous != 0;
@@ -1751,12 +1865,12 @@ struct Runner {
     // elements, so sizes between the powers of two pay off where S divides them
     const int cand[7] = {64, 128, 192, 256, 320, 384, 512};
     for (int k = kExtrap; k <= kRk; k++) {
-      if (k == kFaceBnd && (!visc || st.nBnd == 0)) continue;
+      if ((k == kFaceBnd && (!visc || st.nBnd == 0)) || (k == kRk && !visc)) continue;
       auto run = [&](int nt) {
         switch (k) {
           case kExtrap: launch_extrap(c, nt); break;
           case kFaceInv: launch_face_inv(c, nt); break;
-          case kVolume: launch_volume(c, nt); break;
+          case kVolume: launch_volume(c, nt, visc ? RkArgs<R>{} : rk_args(c, 1, 0.5, 1. / 3., false, 0)); break;
           case kFaceBnd: launch_face_bnd(c, nt); break;
           default: launch_rk(c, nt, 1, 0.5, 1. / 3., false);
         }

@@ -45,7 +45,11 @@ def fused_work(D: int, P: int, E: int, nFF: int, nFFb: int, viscous: bool, nRK: 
         fl += E * N * (D * S * 2 * P1 + D * S * 2 * 2 * D + (D + 1) * F * 2 * P1)
         out["fused_volume_viscous"] = Work((rd + wr) * B8, fl)
     else:
-        out["fused_volume_inviscid"] = Work((rd + wr) * B8, fl)
+        # inviscid runs: the volume kernel also applies the RK update (below);
+        # divF and the discontinuous normal flux stay on chip, the common flux
+        # Fn is read instead
+        vol_rd = rd + E * N * F
+        vol_wr = wr - E * (N * S + N * F)
     if viscous and nFFb:
         # boundary LDG flux: U, normal, gradient in; the point's flux part out
         rd = nFFb * (N + D + D * N + 3 * B4 / B8)
@@ -73,7 +77,12 @@ def fused_work(D: int, P: int, E: int, nFF: int, nFFb: int, viscous: bool, nRK: 
         if st == 0:
             wr += E * N * S                          # divF_spts[0]
         per_stage.append((rd + wr) * B8)
-    out["fused_rk_update"] = Work(sum(per_stage) / len(per_stage), E * N * S * 2 * 2 * D)
+    rk = Work(sum(per_stage) / len(per_stage), E * N * S * 2 * 2 * D)
+    if viscous:
+        out["fused_rk_update"] = rk
+    else:
+        # the RK part without its reads of divF, Fn and disFn
+        out["fused_volume_rk"] = Work((vol_rd + vol_wr) * B8 + rk.bytes - E * (N * S + 2 * N * F) * B8, fl + rk.flops)
     return out
 
 
