@@ -482,6 +482,7 @@ def main() -> int:
         w = work.get(name)
         if w is not None and avg_ms > 0:
             ent["GB/s"] = round(w.bytes / (avg_ms * 1e-3) / 1e9, 1)
+            ent["frac_of_hbm_peak"] = round(w.bytes / (avg_ms * 1e-3) / 1e9 / HBM_PEAK_GBS, 4)
             ent["bytes_per_launch"] = w.bytes
         kernels[name] = ent
     dom = max((k for k in kernels if k in work), key=lambda k: kernels[k]["avg_ms"] * kernels[k]["launches"],
