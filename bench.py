@@ -159,16 +159,16 @@ def batched_form(n_cases: int, seed: int, tol: float, max_steps: int, nx_hi: int
 def served_form(n_req: int, rates: list[float], seed: int) -> dict:
     """Poisson arrivals of the same request mix at each rate in `rates`: 70%
     steady 2-D bump channels (16..128 x 8..48 quads, P 1..5, Mach 0.2-0.5,
-    P-multigrid, residual 1e-8 within 2000 steps), 30% unsteady 3-D periodic
+    P-multigrid, residual 1e-8 within 6000 steps), 30% unsteady 3-D periodic
     Euler vortex columns (8..40^3 hexes, P 2..4, 100 steps); half FP64, half
     mixed precision (mixed steady requests finish in FP64)."""
     from flurrypp_amd.serve import Server, poisson_mix
     out = {"workload": f"{n_req} requests per rate: 70% steady 2-D bump (16..128 x 8..48, P 1..5, residual 1e-8 "
-                       f"within 2000 steps, P-multigrid), 30% 3-D Euler vortex (8..40^3, P 2..4, 100 steps); "
+                       f"within 6000 steps, P-multigrid), 30% 3-D Euler vortex (8..40^3, P 2..4, 100 steps); "
                        f"FP64/mixed 50/50", "rates": {}}
     for rate in rates:
         reqs = poisson_mix(n_req, rate=rate, seed=seed, steady_tol=1e-8, unsteady_steps=100, n3d=(8, 40),
-                           p3d=(2, 4), nx2d=(16, 128), ny2d=(8, 48), p2d=(1, 5), max_steps=2000)
+                           p3d=(2, 4), nx2d=(16, 128), ny2d=(8, 48), p2d=(1, 5), max_steps=6000)
         srv = Server(hbm_budget=200e9, max_active=32, steps_per_tick=10, check_every=50, pmg=True, log=log)
         out["rates"][f"{rate:g}/s"] = srv.run(reqs)
     return out

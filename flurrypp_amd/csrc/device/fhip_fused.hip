@@ -1192,6 +1192,20 @@ __global__ void k_wave_em(int E, const R* __restrict__ Ufe, const double* __rest
   waveSpe[i] = (fabs(vN - 0.) + sqrt(csq));
 }
 
+// Minimum over the block (a multiple of 64 threads, at most 256), returned to
+// every thread: across each wave by lane exchanges, then across the waves
+// through red.  min is exact, so the order of the comparisons does not matter.
+__device__ __forceinline__ double block_min(double v, double* red) {
+#pragma unroll
+  for (int o = 32; o > 0; o >>= 1) v = fmin(v, __shfl_xor(v, o));
+  __syncthreads();  // red may still be read by an earlier call
+  if ((threadIdx.x & 63) == 0) red[threadIdx.x >> 6] = v;
+  __syncthreads();
+  v = red[0];
+  for (int w = 1; w < (int)(blockDim.x >> 6); w++) v = fmin(v, red[w]);
+  return v;
+}
+
 // ele::calcDt per element (element-major inputs) and block minima.  With a
 // ticket counter, the block that finishes last also reduces the block minima
 // to the global dt (solver::calcDt's minimum, src/solver.cpp:447-468) and
@@ -1204,7 +1218,7 @@ __global__ __launch_bounds__(256) void k_ele_dt_em(int E, const double* __restri
                                                     double* __restrict__ out) {
   using Sh = Shape<D, P>;
   constexpr int S = Sh::S, F = Sh::F, Ff = Sh::Ff;
-  __shared__ double red[256];
+  __shared__ double red[4];
   int e = blockIdx.x * blockDim.x + threadIdx.x;
   double dt = INFINITY;
   if (e < E) {
@@ -1215,13 +1229,8 @@ __global__ __launch_bounds__(256) void k_ele_dt_em(int E, const double* __restri
     dt = cflfac * 2 * vol / iw;
     eleDt[e] = dt;
   }
-  red[threadIdx.x] = dt;
-  __syncthreads();
-  for (int st = blockDim.x / 2; st > 0; st >>= 1) {
-    if (threadIdx.x < st) red[threadIdx.x] = fmin(red[threadIdx.x], red[threadIdx.x + st]);
-    __syncthreads();
-  }
-  if (threadIdx.x == 0) part[blockIdx.x] = red[0];
+  const double bmin = block_min(dt, red);
+  if (threadIdx.x == 0) part[blockIdx.x] = bmin;
   if (!ticket) return;
   __shared__ bool last;
   if (threadIdx.x == 0) {
@@ -1234,14 +1243,9 @@ __global__ __launch_bounds__(256) void k_ele_dt_em(int E, const double* __restri
   if (!last) return;
   double v = INFINITY;
   for (int i = threadIdx.x; i < (int)gridDim.x; i += blockDim.x) v = fmin(v, part[i]);
-  red[threadIdx.x] = v;
-  __syncthreads();
-  for (int st = blockDim.x / 2; st > 0; st >>= 1) {
-    if (threadIdx.x < st) red[threadIdx.x] = fmin(red[threadIdx.x], red[threadIdx.x + st]);
-    __syncthreads();
-  }
+  v = block_min(v, red);
   if (threadIdx.x == 0) {
-    *out = red[0];
+    *out = v;
     __hip_atomic_store(ticket, 0u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
   }
 }
