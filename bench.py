@@ -365,6 +365,21 @@ def kinetic_energy(setup, U) -> float:
     return float(np.sum(wj * ke) / np.sum(wj))
 
 
+def kernel_table(kt: dict, work: dict) -> dict:
+    """Per-kernel average time and launches, with the algorithmic bytes per
+    launch, achieved GB/s and fraction of HBM peak where the model has them."""
+    kernels = {}
+    for name, (avg_ms, cnt) in kt.items():
+        ent = {"avg_ms": round(avg_ms, 5), "launches": cnt}
+        w = work.get(name)
+        if w is not None and avg_ms > 0:
+            ent["GB/s"] = round(w.bytes / (avg_ms * 1e-3) / 1e9, 1)
+            ent["frac_of_hbm_peak"] = round(w.bytes / (avg_ms * 1e-3) / 1e9 / HBM_PEAK_GBS, 4)
+            ent["bytes_per_launch"] = w.bytes
+        kernels[name] = ent
+    return kernels
+
+
 def traffic_for(kernel: str, workload: str):
     """HBM bytes per launch from the committed rocprofv3 --pmc summary."""
     p = ROOT / "profiles" / "traffic.json"
@@ -476,15 +491,7 @@ def main() -> int:
     nFFb = setup.faces.nBnd * setup.nFptsFace
     work = perfmodel.fused_work(setup.nDims, setup.order, setup.nEles, nFF, nFFb, bool(visc), nRK,
                                 setup.params.timeType)
-    kernels = {}
-    for name, (avg_ms, cnt) in kt.items():
-        ent = {"avg_ms": round(avg_ms, 5), "launches": cnt}
-        w = work.get(name)
-        if w is not None and avg_ms > 0:
-            ent["GB/s"] = round(w.bytes / (avg_ms * 1e-3) / 1e9, 1)
-            ent["frac_of_hbm_peak"] = round(w.bytes / (avg_ms * 1e-3) / 1e9 / HBM_PEAK_GBS, 4)
-            ent["bytes_per_launch"] = w.bytes
-        kernels[name] = ent
+    kernels = kernel_table(kt, work)
     dom = max((k for k in kernels if k in work), key=lambda k: kernels[k]["avg_ms"] * kernels[k]["launches"],
               default=None)
     roof = None
@@ -525,7 +532,8 @@ def main() -> int:
             "ke_dissipation_rate_fp64": eps64, "ke_dissipation_rate_mixed": eps32,
             "ke_dissipation_rel_err": abs(eps32 - eps64) / abs(eps64) if eps64 else None,
             "after_steps": a.warmup + a.steps,
-            "kernels": {k: {"avg_ms": round(v[0], 5), "launches": v[1]} for k, v in ktm.items()},
+            "kernels": kernel_table(ktm, perfmodel.mixed_work(setup.nDims, setup.order, setup.nEles, nFF, nFFb,
+                                                              bool(visc), nRK, setup.params.timeType)),
         }
 
     # one plain copy of the FP64 state (E*S*N doubles read and written) after

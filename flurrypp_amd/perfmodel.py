@@ -86,6 +86,73 @@ def fused_work(D: int, P: int, E: int, nFF: int, nFFb: int, viscous: bool, nRK: 
     return out
 
 
+def mixed_work(D: int, P: int, E: int, nFF: int, nFFb: int, viscous: bool, nRK: int, time_type: int) -> dict:
+    """Per-launch algorithmic bytes of the mixed-path kernels: FP32 face and
+    volume intermediates, fluxes, U at the flux points, stage copy of U and
+    stored stage residuals, FP32 copies of the geometry; FP64 state, stage-0
+    copy, divF_spts[0], detJ and wave speeds.  Flops as for the FP64 kernels."""
+    N, P1 = D + 2, P + 1
+    S = P1 ** D
+    F = 2 * D * P1 ** (D - 1)
+    nFFi = nFF - nFFb
+    f4, d8, i4 = 4, 8, 4
+    fp = fused_work(D, P, E, nFF, nFFb, viscous, nRK, time_type)
+    out = {}
+    # face: 3 ints, UL, UR, normal, dA L (+R) in; Fn*dA L (+R), wave speed
+    # (FP64) [+ dUc L (+R)] out
+    rd = nFF * (3 * i4 + f4 * (N + D + 1)) + nFFi * f4 * (N + 1)
+    wr = nFF * (f4 * N + d8) + nFFi * f4 * N
+    if viscous:
+        wr += (nFF + nFFi) * f4 * N
+    out["mixed_face_inviscid"] = Work(rd + wr, 0.0)
+    # volume: U (stage copy), metric terms
+    rd = E * f4 * (N * S + D * D * S)
+    if viscous:
+        # 1/detJ, dUc, face normal, dA, side code; divF, this side's flux jump
+        # (with its LDG part), the LDG part, boundary gradient out
+        rd += E * (f4 * (S + N * F + D * F + F) + i4 * F)
+        wr = E * f4 * (N * S + 2 * N * F) + nFFb * f4 * D * N
+        out["mixed_volume_viscous"] = Work(rd + wr, fp["fused_volume_viscous"].flops)
+        if nFFb:
+            # U, normal, gradient, 3 ints, dA in; the point's flux jump updated
+            out["mixed_face_boundary"] = Work(nFFb * (f4 * (N + D + D * N + 1) + 3 * i4 + 2 * f4 * N), 0.0)
+    # classical RK stages keep their residuals in FP32 (the form without a
+    # multigrid source); Jameson-form stages read U0 and write the state last
+    per_stage = []
+    for st in range(nRK):
+        rd_ = E * (f4 * N * S + d8 * S)                # divF, detJ
+        if viscous:
+            rd_ += E * (f4 * 3 * N * F + f4 * F + i4 * F)  # Fn, flux jump, partner's LDG part, dA, code
+        else:
+            rd_ += E * f4 * N * F                      # Fn (the volume kernel forms the jump on chip)
+        wr_ = E * (f4 * N * F + f4 * N * S)            # U at the faces, stage copy of U
+        if time_type >= 5:
+            rd_ += E * d8 * N * S                      # U0
+            if st == nRK - 1:
+                wr_ += E * d8 * N * S                  # U
+        elif nRK == 1:
+            rd_ += E * d8 * N * S
+            wr_ += E * d8 * N * S
+        elif st == 0:
+            rd_ += E * d8 * N * S                      # U
+            wr_ += E * (d8 * N * S + f4 * N * S)       # U0, residual of stage 0
+        elif st == nRK - 1:
+            rd_ += E * (d8 * N * S + f4 * N * S * (nRK - 1))  # U0, stored residuals
+            wr_ += E * d8 * N * S                      # U
+        else:
+            rd_ += E * d8 * N * S                      # U0
+            wr_ += E * f4 * N * S                      # residual of this stage
+        if st == 0:
+            wr_ += E * d8 * N * S                      # divF_spts[0]
+        per_stage.append(rd_ + wr_)
+    rk = sum(per_stage) / len(per_stage)
+    if viscous:
+        out["mixed_rk_update"] = Work(rk, fp["fused_rk_update"].flops)
+    else:
+        out["mixed_volume_rk"] = Work(rd + rk - E * f4 * N * S, fp["fused_volume_rk"].flops)
+    return out
+
+
 def hbm_bound(w: Work, peak_tf: float = 78.6, peak_bw: float = 8.0) -> bool:
     """True when the kernel's intensity is under the FP64 ridge point."""
     if w.bytes == 0:
