@@ -684,6 +684,57 @@ static void pmg_cycle_body(fhip_pmg* mg, const std::vector<char>& fused, const s
   }
 }
 
+// The same cycle when every level runs a fused path of one precision: the
+// levels stay element-major from the fine level's import to the end (no
+// layout conversions around each residual and smoothing pass); the
+// transfers act on the element-major arrays.  The coarse levels' wave speeds
+// come in once (the face kernel rewrites only the left and boundary entries,
+// the dt sums all of them).  At the end every level's U, and the fine level's
+// divF_spts[0] and wave speeds, are written back to FlurryPP's layouts (the
+// coarse levels' residual and the wave speeds the cycle computed stay
+// element-major: the next cycle computes them again before any use).
+static void pmg_cycle_body_em(fhip_pmg* mg, bool mixed) {
+  CycleStream one(mg);
+  const int n = (int)mg->lv.size();
+  fhip::Ctx& fine = mg->lv[n - 1]->c;
+  for (int i = 0; i + 1 < n; i++) {
+    fhip::fused_em_prepare(mg->lv[i]->c, mixed);
+    fhip::fused_em_import_wave(mg->lv[i]->c);
+  }
+  fhip::fused_begin(fine, mixed, false);
+  fhip::fused_em_residual(fine, mixed);
+  auto smooth = [&](fhip::Ctx& g) {
+    fhip::fused_em_set_src(g, true);
+    for (int k = 0; k < mg->smooth; k++) update_step(g, true, mixed, true, false);
+    fhip::fused_em_set_src(g, false);
+  };
+  for (int i = n - 2; i >= 0; i--) {
+    fhip::Ctx& g = mg->lv[i]->c;
+    fhip::Ctx& f = mg->lv[i + 1]->c;
+    const long long nS = g.z.nS();
+    // restriction of U and divF_spts[0]; compute_source_term (src/multigrid.cpp:381-398)
+    fhip::pmg_restrict_em(g, f.z.S, mg->R[i], fhip::fused_em_array(f, 0), fhip::fused_em_array(f, 1), mg->sol[i]);
+    fhip::fused_em_extrap(g, mixed);
+    fhip::fused_em_residual(g, mixed);
+    fhip::sub_array(g, fhip::fused_em_array(g, 2), fhip::fused_em_array(g, 2), fhip::fused_em_array(g, 1), nS);
+    smooth(g);
+    if (i > 0) {
+      fhip::fused_em_residual(g, mixed);
+      fhip::add_array(g, fhip::fused_em_array(g, 1), fhip::fused_em_array(g, 2), nS);
+    }
+  }
+  // upward: smooth again, prolong_err (src/multigrid.cpp:347-364)
+  for (int i = 0; i + 1 < n; i++) {
+    fhip::Ctx& g = mg->lv[i]->c;
+    fhip::Ctx& up = mg->lv[i + 1]->c;
+    smooth(g);
+    fhip::pmg_prolong_em(up, g.z.S, mg->Pro[i], fhip::fused_em_array(g, 0), mg->sol[i]);
+    if (i + 2 < n) fhip::fused_em_extrap(up, mixed);  // the next level up smooths from here
+    fhip::fused_em_export_u(g);
+  }
+  fhip::fused_end(fine);
+}
+
 // The first cycle on a path runs launch by launch (it also allocates what the
 // fused paths allocate on first use); the second is captured into one graph
 // on the fine level's stream, which later cycles replay there.  Timed levels,
@@ -713,8 +764,14 @@ int fhip_pmg_cycle(fhip_pmg* mg, int path) {
       ge = nullptr;
     }
     pmg_settle_coarse(mg);
+    bool em = true;
+    for (int i = 0; i < n; i++) em = em && fused[i] && mixed[i] == mixed[0];
+    auto body = [&] {
+      if (em) pmg_cycle_body_em(mg, mixed[0] != 0);
+      else pmg_cycle_body(mg, fused, mixed);
+    };
     if (!graphs || !mg->warm[path]) {
-      pmg_cycle_body(mg, fused, mixed);
+      body();
       mg->warm[path] = true;
       return 0;
     }
@@ -723,7 +780,7 @@ int fhip_pmg_cycle(fhip_pmg* mg, int path) {
         Flag() { t_capturing_cycle = true; }
         ~Flag() { t_capturing_cycle = false; }
       } flag;
-      ge = capture(fine.stream, [&] { pmg_cycle_body(mg, fused, mixed); });
+      ge = capture(fine.stream, [&] { body(); });
       for (int i = 0; i < n; i++) gen[i] = mg->lv[i]->c.graph_gen;
       mg->gen[path] = gen;
     }

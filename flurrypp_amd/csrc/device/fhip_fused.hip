@@ -1170,6 +1170,63 @@ __global__ void k_copy_em(long long n, double* __restrict__ dst, const double* _
   if (i < n) dst[i] = src[i];
 }
 
+// P-multigrid transfers in element-major layout ([ele][field][point] rows of
+// S values), the products the staged operator GEMM forms in FlurryPP's layout
+// with the same sums (chunks of kc columns of the operator, added in turn).
+// multiGrid::restrict_pmg (src/multigrid.cpp:315-345) and the start of
+// compute_source_term (:381-398): the coarse level's U = R U_fine, its FAS
+// source = R divF_fine (the residual of U is subtracted later) and sol = U.
+__global__ void k_restrict_em(long long rows, int Sc, int Sf, int kc, const double* __restrict__ R,
+                              const double* __restrict__ Uf, const double* __restrict__ divFf,
+                              double* __restrict__ Uc, double* __restrict__ srcc, double* __restrict__ solc) {
+  const long long t = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (t >= rows * Sc) return;
+  const long long r = t / Sc;
+  const int i = (int)(t - r * Sc);
+  const double* a = R + (long long)i * Sf;
+  const double* u = Uf + r * Sf;
+  const double* d = divFf + r * Sf;
+  double cu = 0, cd = 0;
+  for (int l0 = 0; l0 < Sf; l0 += kc) {
+    const int n = min(kc, Sf - l0);
+    double su = 0, sd = 0;
+    for (int l = 0; l < n; l++) {
+      su += a[l0 + l] * u[l0 + l];
+      sd += a[l0 + l] * d[l0 + l];
+    }
+    cu = l0 ? su + cu : su;
+    cd = l0 ? sd + cd : sd;
+  }
+  Uc[t] = cu;
+  solc[t] = cu;
+  srcc[t] = cd;
+}
+
+// prolong_err (src/multigrid.cpp:347-364): U_up += P (U - sol) of the level
+// below.
+__global__ void k_prolong_em(long long rows, int Su, int Sc, int kc, const double* __restrict__ Pm,
+                             const double* __restrict__ Uc, const double* __restrict__ solc, double* __restrict__ Uu) {
+  const long long t = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (t >= rows * Su) return;
+  const long long r = t / Su;
+  const int i = (int)(t - r * Su);
+  const double* a = Pm + (long long)i * Sc;
+  const double* u = Uc + r * Sc;
+  const double* o = solc + r * Sc;
+  double c = Uu[t];
+  for (int l0 = 0; l0 < Sc; l0 += kc) {
+    const int n = min(kc, Sc - l0);
+    double s = 0;
+    for (int l = 0; l < n; l++) {
+      double v = u[l0 + l];
+      v -= o[l0 + l];
+      s += a[l0 + l] * v;
+    }
+    c = s + c;
+  }
+  Uu[t] = c;
+}
+
 // Wave speeds from U at the flux points (ele::calcWaveSpFpts), element-major.
 template <int D, int P, class R>
 __global__ void k_wave_em(int E, const R* __restrict__ Ufe, const double* __restrict__ norme,
@@ -1812,7 +1869,9 @@ struct Runner {
                          st.dAe, cflfac, c.eleDt, c.partial, one ? st.dtTicket : nullptr, c.d_dt);
       FCHK(hipGetLastError());
     }
-    if (st.nMesh > 1) {
+    // merged meshes: per-mesh minima for a global CFL dt per mesh (a local
+    // CFL dt uses the element values alone)
+    if (st.nMesh > 1 && c.prm.dt_type == 1) {
       Launch L(c, nm("fused_dt_min", "mixed_dt_min"));
       hipLaunchKernelGGL(k_mesh_dt_min, dim3(st.nMesh), dim3(256), 0, c.stream, st.meshStart, c.eleDt, st.meshDt);
       FCHK(hipGetLastError());
@@ -2042,6 +2101,69 @@ void fused_begin(Ctx& c, bool mixed, bool src) {
   to_em(c.stream, z.F, z.E, 1, c.waveSp, st.waveSpe);
   FCHK(hipGetLastError());
   o.extrap(c);
+}
+
+// P-multigrid cycle with every level kept element-major (the transfers above;
+// a level's FlurryPP-layout arrays are written back by the caller at the end).
+static int opgemm_chunk(int k) { return k <= 16 ? 16 : (k <= 32 ? 32 : 64); }
+
+void fused_em_prepare(Ctx& c, bool mixed) {
+  FusedState& st = fs(c);
+  get_ops(c, mixed).ensure(c);
+  if (!st.srce) {
+    c.drop_graphs();
+    FCHK(hipMalloc(&st.srce, (size_t)c.z.E * c.z.N * c.z.S * sizeof(double)));
+    poison_fill(st.srce, (size_t)c.z.E * c.z.N * c.z.S * sizeof(double));
+  }
+}
+
+void fused_em_extrap(Ctx& c, bool mixed) { get_ops(c, mixed).extrap(c); }
+
+// The wave speeds from FlurryPP's layout: the face kernel rewrites only the
+// entries of left and boundary flux points (the reference's calcWaveSp
+// stores into the left element); the others keep what the level holds.
+void fused_em_import_wave(Ctx& c) {
+  to_em(c.stream, c.z.F, c.z.E, 1, c.waveSp, fs(c).waveSpe);
+  FCHK(hipGetLastError());
+}
+
+void fused_em_residual(Ctx& c, bool mixed) {
+  Ops o = get_ops(c, mixed);
+  fs(c).use_src = false;
+  o.residual(c);
+  o.rk(c, 5, 0., 0., true, 0);
+}
+
+void fused_em_set_src(Ctx& c, bool on) { fs(c).use_src = on; }
+
+double* fused_em_array(Ctx& c, int which) {
+  FusedState& st = fs(c);
+  return which == 0 ? st.Ue : (which == 1 ? st.divF0e : st.srce);
+}
+
+void fused_em_export_u(Ctx& c) {
+  FusedState& st = fs(c);
+  st.use_src = false;
+  from_em(c.stream, c.z.S, c.z.E, c.z.N, st.Ue, c.U);
+  FCHK(hipGetLastError());
+}
+
+void pmg_restrict_em(Ctx& c, int Sf, const double* R, const double* Uf, const double* divFf, double* solc) {
+  FusedState& st = fs(c);
+  const long long rows = (long long)c.z.E * c.z.N;
+  Launch L(c, "pmg_restrict");
+  hipLaunchKernelGGL(k_restrict_em, dim3(nblk(rows * c.z.S, 256)), dim3(256), 0, c.stream, rows, c.z.S, Sf,
+                     opgemm_chunk(Sf), R, Uf, divFf, st.Ue, st.srce, solc);
+  FCHK(hipGetLastError());
+}
+
+void pmg_prolong_em(Ctx& up, int Sc, const double* Pm, const double* Uc, const double* solc) {
+  FusedState& st = fs(up);
+  const long long rows = (long long)up.z.E * up.z.N;
+  Launch L(up, "pmg_prolong");
+  hipLaunchKernelGGL(k_prolong_em, dim3(nblk(rows * up.z.S, 256)), dim3(256), 0, up.stream, rows, up.z.S, Sc,
+                     opgemm_chunk(Sc), Pm, Uc, solc, st.Ue);
+  FCHK(hipGetLastError());
 }
 
 void fused_end(Ctx& c) {

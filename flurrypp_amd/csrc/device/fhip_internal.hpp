@@ -166,6 +166,21 @@ void fused_release(Ctx& c);
 void fused_begin(Ctx& c, bool mixed, bool src = false);
 void fused_end(Ctx& c);                // export U, divF_spts[0] and wave speeds
 void fused_rk_step(Ctx& c, bool refresh, bool mixed);
+// P-multigrid cycle on element-major levels: work arrays and the FAS source
+// allocated; U at the flux points of the current U; the residual (no source)
+// into divF_spts[0] and the wave speeds; the source on or off for the RK
+// update; U, divF_spts[0] or the source (which 0, 1, 2); U written back to
+// FlurryPP's layout; the transfers into the coarse level c (U, source, sol)
+// and into the upper level up (U += P (U_c - sol_c)).
+void fused_em_prepare(Ctx& c, bool mixed);
+void fused_em_extrap(Ctx& c, bool mixed);
+void fused_em_import_wave(Ctx& c);
+void fused_em_residual(Ctx& c, bool mixed);
+void fused_em_set_src(Ctx& c, bool on);
+double* fused_em_array(Ctx& c, int which);
+void fused_em_export_u(Ctx& c);
+void pmg_restrict_em(Ctx& c, int Sf, const double* R, const double* Uf, const double* divFf, double* solc);
+void pmg_prolong_em(Ctx& up, int Sc, const double* Pm, const double* Uc, const double* solc);
 void fused_residual(Ctx& c, bool mixed);  // divF_spts[0] and wave speeds of the current U
 // block sizes of the fused kernels (extrapolate, face_inviscid, volume, face_boundary, rk)
 void fused_autotune(Ctx& c, bool mixed, int* out);
