@@ -711,17 +711,12 @@ static void pmg_cycle_body_em(fhip_pmg* mg, bool mixed) {
   for (int i = n - 2; i >= 0; i--) {
     fhip::Ctx& g = mg->lv[i]->c;
     fhip::Ctx& f = mg->lv[i + 1]->c;
-    const long long nS = g.z.nS();
     // restriction of U and divF_spts[0]; compute_source_term (src/multigrid.cpp:381-398)
     fhip::pmg_restrict_em(g, f.z.S, mg->R[i], fhip::fused_em_array(f, 0), fhip::fused_em_array(f, 1), mg->sol[i]);
     fhip::fused_em_extrap(g, mixed);
-    fhip::fused_em_residual(g, mixed);
-    fhip::sub_array(g, fhip::fused_em_array(g, 2), fhip::fused_em_array(g, 2), fhip::fused_em_array(g, 1), nS);
+    fhip::fused_em_residual(g, mixed, 1);  // source -= residual
     smooth(g);
-    if (i > 0) {
-      fhip::fused_em_residual(g, mixed);
-      fhip::add_array(g, fhip::fused_em_array(g, 1), fhip::fused_em_array(g, 2), nS);
-    }
+    if (i > 0) fhip::fused_em_residual(g, mixed, 2);  // residual + source, restricted below
   }
   // upward: smooth again, prolong_err (src/multigrid.cpp:347-364)
   for (int i = 0; i + 1 < n; i++) {

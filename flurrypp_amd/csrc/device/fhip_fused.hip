@@ -472,7 +472,9 @@ __global__ __launch_bounds__(kMaxBlock) void k_face_inv(int nFF, const int* __re
 // Correction of the divergence (correctDivFlux), RK update (timeStepA/B
 // folded into one pass per stage, FP64) and U at the flux points for the
 // next stage.  mode: 0 first classical stage, 1 middle classical stage,
-// 2 last classical stage, 3 single-stage forward Euler, 4 Jameson-form stage.
+// 2 last classical stage, 3 single-stage forward Euler, 4 Jameson-form stage,
+// 6 first Jameson-form stage (which also keeps U as the step's U0); 5, 7
+// and 8 the residual alone (see rk_tail_p).
 // Classical stages combine into U0 - sum_j b_j r_j / detJ dt in timeStepB's
 // order (src/solver.cpp:504-536): either through the FP64 accumulator Uacc, or
 // (mixed path, Rst given) from the stored stage residuals, which FP32 holds
@@ -531,8 +533,10 @@ __device__ __forceinline__ void rk_tail_p(int E, const R* dv, R* sU, const R* sD
   const long long nS = (long long)E * N * S;
   const int e0 = blockIdx.x * EB, nb = min(EB, E - e0);
   const int b = threadIdx.x / S, s = threadIdx.x - b * S;
-  if (mode == 5) {
-    // residual only (calcResidual(0) + correctDivFlux): divF_spts[0], no update
+  if (mode == 5 || mode == 7 || mode == 8) {
+    // residual only (calcResidual(0) + correctDivFlux), no update: 5 into
+    // divF_spts[0]; 7 subtracted in place from the array divF0e points to
+    // (the P-multigrid source); 8 plus the source into divF_spts[0]
     if (b < nb) {
       const long long e = e0 + b;
 #pragma unroll
@@ -540,7 +544,15 @@ __device__ __forceinline__ void rk_tail_p(int E, const R* dv, R* sU, const R* sD
         R c = 0;
 #pragma unroll
         for (int j = 0; j < NC; j++) c += tw<R>(tb.cor_w, tb.cor_wf)[s * NC + j] * sD[(b * N + k) * F + tb.cor_idx[s * NC + j]];
-        st_s<NT>(double(c + dv[k]), &divF0e[(e * N + k) * S + s]);
+        const double r0 = double(c + dv[k]);
+        const long long iu = (e * N + k) * S + s;
+        if (mode == 7) {
+          double v = divF0e[iu];
+          v -= r0;
+          divF0e[iu] = v;
+        } else {
+          st_s<NT>(mode == 8 ? r0 + srce[iu] : r0, &divF0e[iu]);
+        }
       }
     }
     return;  // uniform over the block: no barrier follows for anyone
@@ -555,7 +567,7 @@ __device__ __forceinline__ void rk_tail_p(int E, const R* dv, R* sU, const R* sD
     // of the first and the single stage only (and of frozen batch members,
     // which keep it)
     double uin[N], u0in[N], accin[N];
-    const bool needU = mode == 0 || mode == 3 || frozen;
+    const bool needU = mode == 0 || mode == 3 || mode == 6 || frozen;
     const bool needU0 = mode == 1 || mode == 4 || (mode == 2 && Rst);
     const bool needAcc = !Rst && (mode == 1 || mode == 2);
 #pragma unroll
@@ -604,6 +616,9 @@ __device__ __forceinline__ void rk_tail_p(int E, const R* dv, R* sU, const R* sD
         un = acc - bcoef * r / dj * dt;
       } else if (mode == 3) {
         un = u - bcoef * r / dj * dt;
+      } else if (mode == 6) {
+        st_s<NT>(u, &U0e[iu]);
+        un = u - a * r / dj * dt;
       } else {
         un = u0in[k] - a * r / dj * dt;
       }
@@ -1227,6 +1242,80 @@ __global__ void k_prolong_em(long long rows, int Su, int Sc, int kc, const doubl
   Uu[t] = c;
 }
 
+// The same transfers with the operator (transposed, so that the lanes of a
+// wave read consecutive entries) and the block's rows staged in LDS: nr rows
+// per block, one output per thread, the sums in the same order as above.
+// Used where the operator and the rows fit in 64 KiB.
+__global__ void k_restrict_em_lds(long long rows, int Sc, int Sf, int kc, int nr, const double* __restrict__ R,
+                                  const double* __restrict__ Uf, const double* __restrict__ divFf,
+                                  double* __restrict__ Uc, double* __restrict__ srcc, double* __restrict__ solc) {
+  double* const sA = reinterpret_cast<double*>(fhip_lds);  // [Sf][Sc]
+  double* const sU = sA + Sc * Sf;                          // [nr][Sf]
+  double* const sV = sU + nr * Sf;                          // [nr][Sf]
+  const long long r0 = (long long)blockIdx.x * nr;
+  const int nb = (int)min((long long)nr, rows - r0);
+  for (int j = threadIdx.x; j < Sc * Sf; j += blockDim.x) {
+    const int i = j / Sf;
+    sA[(j - i * Sf) * Sc + i] = R[j];
+  }
+  for (int j = threadIdx.x; j < nb * Sf; j += blockDim.x) {
+    sU[j] = Uf[r0 * Sf + j];
+    sV[j] = divFf[r0 * Sf + j];
+  }
+  __syncthreads();
+  const int rr = threadIdx.x / Sc, i = threadIdx.x - rr * Sc;
+  if (rr >= nb) return;
+  const double* u = sU + rr * Sf;
+  const double* d = sV + rr * Sf;
+  double cu = 0, cd = 0;
+  for (int l0 = 0; l0 < Sf; l0 += kc) {
+    const int n = min(kc, Sf - l0);
+    double su = 0, sd = 0;
+    for (int l = 0; l < n; l++) {
+      const double a = sA[(l0 + l) * Sc + i];
+      su += a * u[l0 + l];
+      sd += a * d[l0 + l];
+    }
+    cu = l0 ? su + cu : su;
+    cd = l0 ? sd + cd : sd;
+  }
+  const long long t = r0 * Sc + threadIdx.x;
+  Uc[t] = cu;
+  solc[t] = cu;
+  srcc[t] = cd;
+}
+
+__global__ void k_prolong_em_lds(long long rows, int Su, int Sc, int kc, int nr, const double* __restrict__ Pm,
+                                 const double* __restrict__ Uc, const double* __restrict__ solc,
+                                 double* __restrict__ Uu) {
+  double* const sA = reinterpret_cast<double*>(fhip_lds);  // [Sc][Su]
+  double* const sV = sA + Su * Sc;                          // [nr][Sc]: U - sol
+  const long long r0 = (long long)blockIdx.x * nr;
+  const int nb = (int)min((long long)nr, rows - r0);
+  for (int j = threadIdx.x; j < Su * Sc; j += blockDim.x) {
+    const int i = j / Sc;
+    sA[(j - i * Sc) * Su + i] = Pm[j];
+  }
+  for (int j = threadIdx.x; j < nb * Sc; j += blockDim.x) {
+    double v = Uc[r0 * Sc + j];
+    v -= solc[r0 * Sc + j];
+    sV[j] = v;
+  }
+  __syncthreads();
+  const int rr = threadIdx.x / Su, i = threadIdx.x - rr * Su;
+  if (rr >= nb) return;
+  const double* v = sV + rr * Sc;
+  const long long t = r0 * Su + threadIdx.x;
+  double c = Uu[t];
+  for (int l0 = 0; l0 < Sc; l0 += kc) {
+    const int n = min(kc, Sc - l0);
+    double s = 0;
+    for (int l = 0; l < n; l++) s += sA[(l0 + l) * Su + i] * v[l0 + l];
+    c = s + c;
+  }
+  Uu[t] = c;
+}
+
 // Wave speeds from U at the flux points (ele::calcWaveSpFpts), element-major.
 template <int D, int P, class R>
 __global__ void k_wave_em(int E, const R* __restrict__ Ufe, const double* __restrict__ norme,
@@ -1830,9 +1919,12 @@ struct Runner {
     Work<R>& w = st.work<R>();
     RkB bs{};
     for (int j = 0; j < c.z.nRK && j < 8; j++) bs.b[j] = c.prm.rk_b[j];
+    // residual modes 7 and 8: the source is the array written, or read
+    double* d0 = mode == 7 ? st.srce : (write0 ? st.divF0e : nullptr);
+    const double* src = mode == 8 ? st.srce : (st.use_src && mode != 7 ? st.srce : nullptr);
     return RkArgs<R>{mode, a, b, st.detJe, c.d_dt, c.eleDt,
                      c.prm.dt_type == 2 ? 1 : (c.prm.dt_type == 1 && st.nMesh > 1 ? 2 : 0), st.Ue, st.U0e, st.Uacc,
-                     write0 ? st.divF0e : nullptr, st.eleMesh, st.meshDt, st.active, st.use_src ? st.srce : nullptr,
+                     d0, st.eleMesh, st.meshDt, st.active, src,
                      (w.Rst && !st.use_src) ? w.Rst : nullptr, stage, bs, w.Us,
                      (mode == 2 || mode == 3 || (mode == 4 && stage == c.z.nRK - 1)) ? 1 : 0, w.Ufe};
   }
@@ -2106,6 +2198,8 @@ void fused_begin(Ctx& c, bool mixed, bool src) {
 // P-multigrid cycle with every level kept element-major (the transfers above;
 // a level's FlurryPP-layout arrays are written back by the caller at the end).
 static int opgemm_chunk(int k) { return k <= 16 ? 16 : (k <= 32 ? 32 : 64); }
+// LDS the staged transfers may take (operator plus the block's rows)
+constexpr size_t kTransferLds = 64 * 1024;
 
 void fused_em_prepare(Ctx& c, bool mixed) {
   FusedState& st = fs(c);
@@ -2127,11 +2221,11 @@ void fused_em_import_wave(Ctx& c) {
   FCHK(hipGetLastError());
 }
 
-void fused_em_residual(Ctx& c, bool mixed) {
+void fused_em_residual(Ctx& c, bool mixed, int how) {
   Ops o = get_ops(c, mixed);
   fs(c).use_src = false;
   o.residual(c);
-  o.rk(c, 5, 0., 0., true, 0);
+  o.rk(c, how == 1 ? 7 : (how == 2 ? 8 : 5), 0., 0., true, 0);
 }
 
 void fused_em_set_src(Ctx& c, bool on) { fs(c).use_src = on; }
@@ -2152,8 +2246,14 @@ void pmg_restrict_em(Ctx& c, int Sf, const double* R, const double* Uf, const do
   FusedState& st = fs(c);
   const long long rows = (long long)c.z.E * c.z.N;
   Launch L(c, "pmg_restrict");
-  hipLaunchKernelGGL(k_restrict_em, dim3(nblk(rows * c.z.S, 256)), dim3(256), 0, c.stream, rows, c.z.S, Sf,
-                     opgemm_chunk(Sf), R, Uf, divFf, st.Ue, st.srce, solc);
+  const int Sc = c.z.S, nr = 256 / Sc;
+  const size_t lds = ((size_t)Sc * Sf + 2 * (size_t)nr * Sf) * sizeof(double);
+  if (nr >= 1 && lds <= kTransferLds)
+    hipLaunchKernelGGL(k_restrict_em_lds, dim3((unsigned)((rows + nr - 1) / nr)), dim3(nr * Sc), lds, c.stream,
+                       rows, Sc, Sf, opgemm_chunk(Sf), nr, R, Uf, divFf, st.Ue, st.srce, solc);
+  else
+    hipLaunchKernelGGL(k_restrict_em, dim3(nblk(rows * Sc, 256)), dim3(256), 0, c.stream, rows, Sc, Sf,
+                       opgemm_chunk(Sf), R, Uf, divFf, st.Ue, st.srce, solc);
   FCHK(hipGetLastError());
 }
 
@@ -2161,8 +2261,14 @@ void pmg_prolong_em(Ctx& up, int Sc, const double* Pm, const double* Uc, const d
   FusedState& st = fs(up);
   const long long rows = (long long)up.z.E * up.z.N;
   Launch L(up, "pmg_prolong");
-  hipLaunchKernelGGL(k_prolong_em, dim3(nblk(rows * up.z.S, 256)), dim3(256), 0, up.stream, rows, up.z.S, Sc,
-                     opgemm_chunk(Sc), Pm, Uc, solc, st.Ue);
+  const int Su = up.z.S, nr = 256 / Su;
+  const size_t lds = ((size_t)Su * Sc + (size_t)nr * Sc) * sizeof(double);
+  if (nr >= 1 && lds <= kTransferLds)
+    hipLaunchKernelGGL(k_prolong_em_lds, dim3((unsigned)((rows + nr - 1) / nr)), dim3(nr * Su), lds, up.stream,
+                       rows, Su, Sc, opgemm_chunk(Sc), nr, Pm, Uc, solc, st.Ue);
+  else
+    hipLaunchKernelGGL(k_prolong_em, dim3(nblk(rows * Su, 256)), dim3(256), 0, up.stream, rows, Su, Sc,
+                       opgemm_chunk(Sc), Pm, Uc, solc, st.Ue);
   FCHK(hipGetLastError());
 }
 
@@ -2213,16 +2319,11 @@ void fused_rk_step(Ctx& c, bool refresh, bool mixed) {
     if (refresh) o.wave(c);
     o.dt(c, cflfac);
   }
-  if (jameson) {
-    long long nS = (long long)c.z.E * c.z.N * c.z.S;
-    Launch L(c, "fused_copy_u0");
-    hipLaunchKernelGGL(k_copy_em, dim3(nblk(nS, 256)), dim3(256), 0, c.stream, nS, st.U0e, st.Ue);
-    FCHK(hipGetLastError());
-  }
   for (int step = 0; step < nRK; step++) {
     o.residual(c);
+    // the Jameson form's first stage also stores the step's U0
     int mode;
-    if (jameson) mode = 4;
+    if (jameson) mode = step == 0 ? 6 : 4;
     else mode = (step == 0) ? 0 : (step == nRK - 1 ? 2 : 1);
     o.rk(c, mode, A[step + 1], B[step], step == 0, step);
   }

@@ -168,14 +168,16 @@ void fused_end(Ctx& c);                // export U, divF_spts[0] and wave speeds
 void fused_rk_step(Ctx& c, bool refresh, bool mixed);
 // P-multigrid cycle on element-major levels: work arrays and the FAS source
 // allocated; U at the flux points of the current U; the residual (no source)
-// into divF_spts[0] and the wave speeds; the source on or off for the RK
+// and the wave speeds, the residual written (how) 0: into divF_spts[0],
+// 1: subtracted from the source in place, 2: plus the source into
+// divF_spts[0]; the source on or off for the RK
 // update; U, divF_spts[0] or the source (which 0, 1, 2); U written back to
 // FlurryPP's layout; the transfers into the coarse level c (U, source, sol)
 // and into the upper level up (U += P (U_c - sol_c)).
 void fused_em_prepare(Ctx& c, bool mixed);
 void fused_em_extrap(Ctx& c, bool mixed);
 void fused_em_import_wave(Ctx& c);
-void fused_em_residual(Ctx& c, bool mixed);
+void fused_em_residual(Ctx& c, bool mixed, int how = 0);
 void fused_em_set_src(Ctx& c, bool on);
 double* fused_em_array(Ctx& c, int which);
 void fused_em_export_u(Ctx& c);
