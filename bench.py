@@ -122,7 +122,7 @@ def batched_form(n_cases: int, seed: int, tol: float, max_steps: int, nx_hi: int
     t = time.perf_counter()
     b = BatchedSolver(ms, pmg=True, low_order=0, smooth_steps=2, autotune=True)
     setup_s = time.perf_counter() - t
-    tuned = {f"P{g}/level P{q}": v for (g, q), v in sorted(b.block_sizes.items())}
+    tuned = {(f"P{g}/level P{q}" if g else f"merged level P{q}"): v for (g, q), v in sorted(b.block_sizes.items())}
     def progress(step, nconv, ndiv):
         if step % 500 == 0:
             log(f"bench: batched step {step}: {nconv} converged, {ndiv} diverged")

@@ -115,6 +115,8 @@ def hip_lib() -> C.CDLL:
         lib.fhip_set_block_sizes.argtypes = [C.c_void_p, C.c_int, _ip]
         lib.fhip_pmg_create.argtypes = [C.POINTER(C.c_void_p), C.c_int, C.POINTER(_dp), C.POINTER(_dp), C.c_int,
                                         C.POINTER(C.c_void_p)]
+        lib.fhip_pmg_create_batch.argtypes = [C.POINTER(C.c_void_p), C.c_int, C.POINTER(C.c_void_p), C.c_int,
+                                              C.POINTER(_dp), C.POINTER(_dp), C.c_int, C.POINTER(C.c_void_p)]
         lib.fhip_pmg_cycle.argtypes = [C.c_void_p, C.c_int]
         lib.fhip_pmg_destroy.argtypes = [C.c_void_p]
         lib.fhip_pmg_destroy.restype = None

@@ -35,15 +35,20 @@ class Member:
 _PER_MESH = ("n_eles", "rho_bound", "u_bound", "v_bound", "w_bound", "p_bound")
 
 
-def context_key(setup) -> tuple:
+# fhip_params fields that follow from the order
+_PER_ORDER = ("order", "n_spts", "n_fpts", "n_fpts_face")
+
+
+def context_key(setup, with_order: bool = True) -> tuple:
     """What one context takes from a case besides its mesh and free stream:
     element type and order, the operators' point set and correction scheme,
     and every other fhip_params field (dt only where it is fixed).  Cases
-    merge into one context only when these are equal."""
+    merge into one context only when these are equal.  with_order=False
+    leaves out the order and what follows from it."""
     p, fp = setup.params, setup.fp
-    vals = [setup.nDims, setup.order, str(p.sptsTypeQuad).lower(), p.vcjhSchemeQuad]
+    vals = [setup.nDims, setup.order if with_order else None, str(p.sptsTypeQuad).lower(), p.vcjhSchemeQuad]
     for name, _ in nat.FhipParams._fields_:
-        if name in _PER_MESH or (name == "dt" and fp.dt_type != 0):
+        if name in _PER_MESH or (name == "dt" and fp.dt_type != 0) or (not with_order and name in _PER_ORDER):
             continue
         v = getattr(fp, name)
         vals.append(tuple(v) if isinstance(v, C.Array) else v)
@@ -167,12 +172,16 @@ class BatchGroup:
 
 class BatchedSolver:
     """Independent cases grouped by order; one context per order group.  With
-    pmg=True every group of order P > low_order also gets coarse groups of
+    pmg=True every group of order P > low_order also gets coarse levels of
     orders low_order..P-1 (the same meshes) and a P-multigrid cycle after each
-    of its updates."""
+    of its updates.  merged_cycle=True (the default, where the groups differ
+    in order alone and run a fused path) merges the coarse levels of one
+    order across the groups and runs one batched cycle
+    (fhip_pmg_create_batch) for all of them; False gives every group its own
+    coarse levels and cycle.  Both give the same results bit for bit."""
 
     def __init__(self, members: list[Member], path: int = nat.PATH_FUSED, pmg: bool = False, low_order: int = 0,
-                 smooth_steps: int = 1, autotune: bool = False):
+                 smooth_steps: int = 1, autotune: bool = False, merged_cycle: bool = True):
         self.members = members
         self.path = path
         self.pmg = pmg
@@ -198,7 +207,15 @@ class BatchedSolver:
         self.coarse: list[list[BatchGroup]] = [[] for _ in self.groups]
         self._mg: list = [None] * len(self.groups)
         self._ops = []
-        if pmg:
+        # batched cycle: fine groups above low_order (ascending order) and one
+        # merged coarse level per order below the highest
+        self.mcoarse: list[BatchGroup] = []
+        self._mfine: list[int] = []
+        self._mgb = None
+        fam = {context_key(g.m.members[0].setup, with_order=False) for g in self.groups if g.m.order > low_order}
+        if pmg and merged_cycle and len(fam) == 1 and path in (nat.PATH_FUSED, nat.PATH_MIXED):
+            self._build_merged_cycle(low_order, smooth_steps)
+        elif pmg:
             for gi, g in enumerate(self.groups):
                 P = g.m.order
                 if P <= low_order:
@@ -226,16 +243,54 @@ class BatchedSolver:
                 h = C.c_void_p()
                 nat.hip_check(nat.hip_lib().fhip_pmg_create(hs_, n, Ra, Pa, smooth_steps, C.byref(h)))
                 self._mg[gi] = h
-        # block sizes of every context's kernels, timed on its own data
+        # block sizes of every context's kernels, timed on its own data;
+        # merged coarse levels under the key (0, order)
         self.block_sizes = {}
         if autotune and path in (nat.PATH_FUSED, nat.PATH_MIXED):
-            for gi, g in enumerate(self.groups):
-                for lv in self.coarse[gi] + [g]:
-                    out = np.zeros(5, dtype=np.int32)
-                    nat.hip_check(nat.hip_lib().fhip_autotune(lv._ctx, path, nat.iptr(out)))
-                    self.block_sizes[(g.m.order, lv.m.order)] = out.tolist()
+            tuned = [(g.m.order, lv) for gi, g in enumerate(self.groups) for lv in self.coarse[gi] + [g]]
+            tuned += [(0, lv) for lv in self.mcoarse]
+            for key, lv in tuned:
+                out = np.zeros(5, dtype=np.int32)
+                nat.hip_check(nat.hip_lib().fhip_autotune(lv._ctx, path, nat.iptr(out)))
+                self.block_sizes[(key, lv.m.order)] = out.tolist()
+
+    def _build_merged_cycle(self, low_order: int, smooth_steps: int) -> None:
+        fine = [gi for gi, g in enumerate(self.groups) if g.m.order > low_order]
+        if not fine:
+            return
+        self._mfine = fine
+        top = self.groups[fine[-1]].m.order
+        for q in range(low_order, top):
+            cm = []
+            for gi in fine:
+                if self.groups[gi].m.order <= q:
+                    continue
+                for m in self.groups[gi].m.members:
+                    pc = copy.copy(m.setup.params)
+                    pc.order = q
+                    hs = HostSetup(pc, m.setup.mesh)
+                    cm.append(Member(hs, hs.initial_condition(), m.tag))
+            cg = BatchGroup(MergedSetup(cm))
+            cg.initialize()
+            self.mcoarse.append(cg)
+        g0 = self.groups[fine[0]].m
+        D, sp = g0.nDims, g0.params.sptsTypeQuad
+        R = [pmg_operators(D, q + 1, sp)[1] for q in range(low_order, top)]
+        Pr = [pmg_operators(D, q, sp)[0] for q in range(low_order, top)]
+        self._ops.append((R, Pr))
+        nf, nc = len(fine), len(self.mcoarse)
+        fa = (C.c_void_p * nf)(*[self.groups[gi]._ctx.value for gi in fine])
+        ca = (C.c_void_p * nc)(*[cg._ctx.value for cg in self.mcoarse])
+        Ra = (nat._dp * nc)(*[nat.dptr(a) for a in R])
+        Pa = (nat._dp * nc)(*[nat.dptr(a) for a in Pr])
+        h = C.c_void_p()
+        nat.hip_check(nat.hip_lib().fhip_pmg_create_batch(fa, nf, ca, nc, Ra, Pa, smooth_steps, C.byref(h)))
+        self._mgb = h
 
     def close(self):
+        if self._mgb is not None and self._mgb.value:
+            nat.hip_lib().fhip_pmg_destroy(self._mgb)
+        self._mgb = None
         for h in self._mg:
             if h is not None and h.value:
                 nat.hip_lib().fhip_pmg_destroy(h)
@@ -243,13 +298,19 @@ class BatchedSolver:
         for cs in self.coarse:
             for g in cs:
                 g.close()
+        for g in self.mcoarse:
+            g.close()
         for g in self.groups:
             g.close()
 
     def step(self, n: int = 1) -> None:
         arr = (C.c_void_p * len(self.groups))(*[g._ctx.value for g in self.groups])
         lib = nat.hip_lib()
-        if not any(h is not None for h in self._mg):
+        if self._mgb is not None:
+            for _ in range(n):
+                nat.hip_check(lib.fhip_update_many(arr, len(self.groups), 1, self.path))
+                nat.hip_check(lib.fhip_pmg_cycle(self._mgb, self.path))
+        elif not any(h is not None for h in self._mg):
             nat.hip_check(lib.fhip_update_many(arr, len(self.groups), n, self.path))
         else:
             for _ in range(n):
@@ -303,12 +364,18 @@ class BatchedSolver:
             stopped = self.converged | diverged
             if progress is not None:
                 progress(done, int(self.converged.sum()), int(diverged.sum()))
+            acts = []
             for gi, g in enumerate(self.groups):
                 act = np.array([0 if stopped[i] else 1 for i, (gg, _) in enumerate(self.index) if gg == gi],
                                dtype=np.int32)
+                acts.append(act)
                 g.set_active(act)
                 for cg in self.coarse[gi]:
                     cg.set_active(act)
+            # a merged coarse level holds the groups above its order, ascending
+            for cg in self.mcoarse:
+                cg.set_active(np.concatenate([acts[gi] for gi in self._mfine
+                                              if self.groups[gi].m.order > cg.m.order]))
         return {"steps": steps, "residual": last, "converged": self.converged.copy(), "diverged": diverged,
                 "time_s": t_conv,
                 "wall_s": time.perf_counter() - t0, "dof_stages": work}

@@ -538,14 +538,26 @@ struct fhip_pmg {
   hipGraphExec_t graph[6] = {};  // by path
   std::vector<unsigned long long> gen[6];
   bool warm[6] = {};
+  // batched form (fhip_pmg_create_batch): lv holds the merged coarse levels,
+  // fine the fine contexts; coarse level j takes its first splitE[j]
+  // elements from fine context fineOf[j] (-1: none) and the rest from level
+  // j + 1.  ev[k], k >= 1: fine context k's queued work; ev[0]: a cycle's end.
+  std::vector<fhip_ctx*> fine;
+  std::vector<int> fineOf;
+  std::vector<long long> splitE;
+  std::vector<hipEvent_t> ev;
+  // the stream a cycle runs on
+  fhip_ctx* owner() const { return fine.empty() ? lv.back() : fine[0]; }
 };
 
 static void pmg_free(fhip_pmg* mg) {
   if (!mg) return;
-  if (!mg->lv.empty()) (void)hipSetDevice(mg->lv.back()->c.device);
-  if (!mg->lv.empty()) (void)hipStreamSynchronize(mg->lv.back()->c.stream);
+  if (!mg->lv.empty()) (void)hipSetDevice(mg->owner()->c.device);
+  if (!mg->lv.empty()) (void)hipStreamSynchronize(mg->owner()->c.stream);
   for (hipGraphExec_t g : mg->graph)
     if (g) (void)hipGraphExecDestroy(g);
+  for (hipEvent_t e : mg->ev)
+    if (e) (void)hipEventDestroy(e);
   for (auto* v : {&mg->R, &mg->Pro, &mg->sol, &mg->corr})
     for (double* p : *v)
       if (p) (void)hipFree(p);
@@ -597,20 +609,88 @@ int fhip_pmg_create(fhip_ctx* const* levels, int n_levels, const double* const* 
 
 void fhip_pmg_destroy(fhip_pmg* mg) { pmg_free(mg); }
 
-// While a cycle is queued every level launches on the fine level's stream;
-// the coarse levels get their own streams back at scope exit.
+int fhip_pmg_create_batch(fhip_ctx* const* fine, int n_fine, fhip_ctx* const* coarse, int n_coarse,
+                          const double* const* opp_restrict, const double* const* opp_prolong, int smooth_steps,
+                          fhip_pmg** out) {
+  fhip_pmg* mg = nullptr;
+  int rc = guard([&] {
+    require(fine && coarse && opp_restrict && opp_prolong && out, "null argument");
+    require(n_fine >= 1 && n_coarse >= 1, "a batched P-multigrid needs fine and coarse levels");
+    require(smooth_steps >= 0, "smoothSteps must be >= 0");
+    for (int k = 0; k < n_fine; k++) require(fine[k] != nullptr, "null fine level");
+    for (int j = 0; j < n_coarse; j++) require(coarse[j] != nullptr, "null coarse level");
+    const fhip::Ctx& c0 = coarse[0]->c;
+    const int low = c0.z.P;
+    for (int k = 0; k < n_fine; k++) {
+      const fhip::Ctx& f = fine[k]->c;
+      require(f.z.N == c0.z.N && f.z.D == c0.z.D && f.device == c0.device,
+              "batched P-multigrid levels must share the equation, the dimension and the device");
+      require(f.z.P > low && f.z.P <= low + n_coarse, "fine orders must lie above the coarse levels'");
+      require(k == 0 || f.z.P > fine[k - 1]->c.z.P, "fine orders must ascend");
+    }
+    require(fine[n_fine - 1]->c.z.P == low + n_coarse, "the top coarse level must be one below the highest fine order");
+    std::vector<int> of(n_coarse, -1);
+    for (int k = 0; k < n_fine; k++) of[fine[k]->c.z.P - low - 1] = k;
+    for (int j = 0; j < n_coarse; j++) {
+      const fhip::Ctx& c = coarse[j]->c;
+      require(c.z.N == c0.z.N && c.z.D == c0.z.D && c.device == c0.device,
+              "batched P-multigrid levels must share the equation, the dimension and the device");
+      require(c.z.P == low + j, "coarse level orders must ascend by one");
+      require(opp_restrict[j] && opp_prolong[j], "null transfer operator");
+      const long long eA = of[j] >= 0 ? fine[of[j]]->c.z.E : 0;
+      const long long eB = j + 1 < n_coarse ? coarse[j + 1]->c.z.E : 0;
+      require(c.z.E == eA + eB, "a coarse level must hold the meshes of the fine level one order up and of the "
+                                "coarse level above it");
+    }
+    HCHK(hipSetDevice(c0.device));
+    mg = new fhip_pmg();
+    mg->lv.assign(coarse, coarse + n_coarse);
+    mg->fine.assign(fine, fine + n_fine);
+    mg->fineOf = of;
+    mg->smooth = smooth_steps;
+    for (int j = 0; j < n_coarse; j++) {
+      fhip::Ctx& c = coarse[j]->c;
+      // the level one order up: the coarse level above, or at the top the fine one
+      const size_t Sc = c.z.S, nS = c.z.nS();
+      const size_t Sf = j + 1 < n_coarse ? coarse[j + 1]->c.z.S : fine[of[j]]->c.z.S;
+      mg->splitE.push_back(of[j] >= 0 ? fine[of[j]]->c.z.E : 0);
+      mg->R.push_back(dalloc<double>(Sc * Sf));
+      HCHK(hipMemcpy(mg->R.back(), opp_restrict[j], Sc * Sf * sizeof(double), hipMemcpyHostToDevice));
+      mg->Pro.push_back(dalloc<double>(Sc * Sf));
+      HCHK(hipMemcpy(mg->Pro.back(), opp_prolong[j], Sc * Sf * sizeof(double), hipMemcpyHostToDevice));
+      mg->sol.push_back(dalloc<double>(nS));
+      if (!c.src) c.src = dalloc<double>(nS);
+    }
+    mg->ev.assign(n_fine, nullptr);
+    for (int k = 0; k < n_fine; k++) HCHK(hipEventCreateWithFlags(&mg->ev[k], hipEventDisableTiming));
+    *out = mg;
+    return 0;
+  });
+  if (rc != 0) {
+    pmg_free(mg);
+    if (out) *out = nullptr;
+  }
+  return rc;
+}
+
+// While a cycle is queued every level launches on the fine level's stream
+// (batched form: fine context 0's); the others get their own streams back at
+// scope exit.
 struct CycleStream {
   fhip_pmg* mg;
+  std::vector<fhip_ctx*> all;
   std::vector<hipStream_t> own;
   explicit CycleStream(fhip_pmg* m) : mg(m) {
-    hipStream_t fine = mg->lv.back()->c.stream;
-    for (fhip_ctx* h : mg->lv) {
+    hipStream_t st = mg->owner()->c.stream;
+    all = mg->lv;
+    all.insert(all.end(), mg->fine.begin(), mg->fine.end());
+    for (fhip_ctx* h : all) {
       own.push_back(h->c.stream);
-      h->c.stream = fine;
+      h->c.stream = st;
     }
   }
   ~CycleStream() {
-    for (size_t i = 0; i < own.size(); i++) mg->lv[i]->c.stream = own[i];
+    for (size_t i = 0; i < own.size(); i++) all[i]->c.stream = own[i];
   }
 };
 
@@ -621,7 +701,7 @@ struct CycleStream {
 // hardware queue).
 static void pmg_settle_coarse(fhip_pmg* mg) {
   const int n = (int)mg->lv.size();
-  for (int i = 0; i + 1 < n; i++) {
+  for (int i = 0; i + (mg->fine.empty() ? 1 : 0) < n; i++) {
     hipStream_t coarse = mg->lv[i]->c.stream;
     const hipError_t q = hipStreamQuery(coarse);
     if (q == hipErrorNotReady) HCHK(hipStreamSynchronize(coarse));
@@ -712,7 +792,8 @@ static void pmg_cycle_body_em(fhip_pmg* mg, bool mixed) {
     fhip::Ctx& g = mg->lv[i]->c;
     fhip::Ctx& f = mg->lv[i + 1]->c;
     // restriction of U and divF_spts[0]; compute_source_term (src/multigrid.cpp:381-398)
-    fhip::pmg_restrict_em(g, f.z.S, mg->R[i], fhip::fused_em_array(f, 0), fhip::fused_em_array(f, 1), mg->sol[i]);
+    fhip::pmg_restrict_em(g, 0, g.z.E, f.z.S, mg->R[i], fhip::fused_em_array(f, 0), fhip::fused_em_array(f, 1),
+                          mg->sol[i]);
     fhip::fused_em_extrap(g, mixed);
     fhip::fused_em_residual(g, mixed, 1);  // source -= residual
     smooth(g);
@@ -730,6 +811,61 @@ static void pmg_cycle_body_em(fhip_pmg* mg, bool mixed) {
   fhip::fused_end(fine);
 }
 
+// The batched cycle: every mesh goes through the steps of
+// pmg_cycle_body_em in the same order; a merged coarse level's steps run once
+// for all of its meshes.  Down the levels a coarse level restricts its first
+// elements from the fine context one order up and the rest from the coarse
+// level above; up the levels it prolongs into the same two.
+static void pmg_cycle_body_batch(fhip_pmg* mg, bool mixed) {
+  CycleStream one(mg);
+  const int nc = (int)mg->lv.size();
+  for (int j = 0; j < nc; j++) {
+    fhip::fused_em_prepare(mg->lv[j]->c, mixed);
+    fhip::fused_em_import_wave(mg->lv[j]->c);
+  }
+  for (fhip_ctx* h : mg->fine) {
+    fhip::fused_begin(h->c, mixed, false);
+    fhip::fused_em_residual(h->c, mixed);
+  }
+  auto smooth = [&](fhip::Ctx& g) {
+    fhip::fused_em_set_src(g, true);
+    for (int k = 0; k < mg->smooth; k++) update_step(g, true, mixed, true, false);
+    fhip::fused_em_set_src(g, false);
+  };
+  for (int j = nc - 1; j >= 0; j--) {
+    fhip::Ctx& g = mg->lv[j]->c;
+    const long long eA = mg->splitE[j];
+    if (mg->fineOf[j] >= 0) {
+      fhip::Ctx& f = mg->fine[mg->fineOf[j]]->c;
+      fhip::pmg_restrict_em(g, 0, eA, f.z.S, mg->R[j], fhip::fused_em_array(f, 0), fhip::fused_em_array(f, 1),
+                            mg->sol[j]);
+    }
+    if (j + 1 < nc) {
+      fhip::Ctx& f = mg->lv[j + 1]->c;
+      fhip::pmg_restrict_em(g, eA, g.z.E - eA, f.z.S, mg->R[j], fhip::fused_em_array(f, 0),
+                            fhip::fused_em_array(f, 1), mg->sol[j]);
+    }
+    fhip::fused_em_extrap(g, mixed);
+    fhip::fused_em_residual(g, mixed, 1);  // source -= residual
+    smooth(g);
+    if (j > 0) fhip::fused_em_residual(g, mixed, 2);  // residual + source, restricted below
+  }
+  for (int j = 0; j < nc; j++) {
+    fhip::Ctx& g = mg->lv[j]->c;
+    const long long offA = mg->splitE[j] * g.z.N * g.z.S;
+    smooth(g);
+    if (mg->fineOf[j] >= 0)
+      fhip::pmg_prolong_em(mg->fine[mg->fineOf[j]]->c, g.z.S, mg->Pro[j], fhip::fused_em_array(g, 0), mg->sol[j]);
+    if (j + 1 < nc) {
+      fhip::Ctx& up = mg->lv[j + 1]->c;
+      fhip::pmg_prolong_em(up, g.z.S, mg->Pro[j], fhip::fused_em_array(g, 0) + offA, mg->sol[j] + offA);
+      fhip::fused_em_extrap(up, mixed);  // the next level up smooths from here
+    }
+    fhip::fused_em_export_u(g);
+  }
+  for (fhip_ctx* h : mg->fine) fhip::fused_end(h->c);
+}
+
 // The first cycle on a path runs launch by launch (it also allocates what the
 // fused paths allocate on first use); the second is captured into one graph
 // on the fine level's stream, which later cycles replay there.  Timed levels,
@@ -738,35 +874,54 @@ static void pmg_cycle_body_em(fhip_pmg* mg, bool mixed) {
 int fhip_pmg_cycle(fhip_pmg* mg, int path) {
   return guard([&] {
     require(mg, "null multigrid");
-    const int n = (int)mg->lv.size();
+    std::vector<fhip_ctx*> all = mg->lv;
+    all.insert(all.end(), mg->fine.begin(), mg->fine.end());
+    const int n = (int)all.size();
     std::vector<char> fused(n), mixed(n);
     bool graphs = true;
     std::vector<unsigned long long> gen(n);
     for (int i = 0; i < n; i++) {
       bool m;
-      fhip::Ctx& c = mg->lv[i]->c;
+      fhip::Ctx& c = all[i]->c;
       fused[i] = resolve_path(c, path, m);
       mixed[i] = m;
       graphs = graphs && c.use_graphs && !c.timing;
       gen[i] = c.graph_gen;
     }
-    fhip::Ctx& fine = mg->lv[n - 1]->c;
-    HCHK(hipSetDevice(fine.device));
+    fhip::Ctx& own = mg->owner()->c;
+    HCHK(hipSetDevice(own.device));
     hipGraphExec_t& ge = mg->graph[path];
     if (ge && (!graphs || mg->gen[path] != gen)) {
-      HCHK(hipStreamSynchronize(fine.stream));
+      HCHK(hipStreamSynchronize(own.stream));
       (void)hipGraphExecDestroy(ge);
       ge = nullptr;
     }
     pmg_settle_coarse(mg);
     bool em = true;
     for (int i = 0; i < n; i++) em = em && fused[i] && mixed[i] == mixed[0];
+    const bool batch = !mg->fine.empty();
+    require(!batch || em, "a batched P-multigrid runs on a fused path of one precision");
     auto body = [&] {
-      if (em) pmg_cycle_body_em(mg, mixed[0] != 0);
+      if (batch) pmg_cycle_body_batch(mg, mixed[0] != 0);
+      else if (em) pmg_cycle_body_em(mg, mixed[0] != 0);
       else pmg_cycle_body(mg, fused, mixed);
+    };
+    // batched form: the cycle follows the work queued on every fine
+    // context's stream, and their later work follows the cycle
+    if (batch) {
+      for (size_t k = 1; k < mg->fine.size(); k++) {
+        HCHK(hipEventRecord(mg->ev[k], mg->fine[k]->c.stream));
+        HCHK(hipStreamWaitEvent(own.stream, mg->ev[k], 0));
+      }
+    }
+    auto join = [&] {
+      if (!batch || mg->fine.size() < 2) return;
+      HCHK(hipEventRecord(mg->ev[0], own.stream));
+      for (size_t k = 1; k < mg->fine.size(); k++) HCHK(hipStreamWaitEvent(mg->fine[k]->c.stream, mg->ev[0], 0));
     };
     if (!graphs || !mg->warm[path]) {
       body();
+      join();
       mg->warm[path] = true;
       return 0;
     }
@@ -775,11 +930,12 @@ int fhip_pmg_cycle(fhip_pmg* mg, int path) {
         Flag() { t_capturing_cycle = true; }
         ~Flag() { t_capturing_cycle = false; }
       } flag;
-      ge = capture(fine.stream, [&] { body(); });
-      for (int i = 0; i < n; i++) gen[i] = mg->lv[i]->c.graph_gen;
+      ge = capture(own.stream, [&] { body(); });
+      for (int i = 0; i < n; i++) gen[i] = all[i]->c.graph_gen;
       mg->gen[path] = gen;
     }
-    HCHK(hipGraphLaunch(ge, fine.stream));
+    HCHK(hipGraphLaunch(ge, own.stream));
+    join();
     return 0;
   });
 }

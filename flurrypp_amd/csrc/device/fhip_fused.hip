@@ -2242,18 +2242,22 @@ void fused_em_export_u(Ctx& c) {
   FCHK(hipGetLastError());
 }
 
-void pmg_restrict_em(Ctx& c, int Sf, const double* R, const double* Uf, const double* divFf, double* solc) {
+void pmg_restrict_em(Ctx& c, long long e0, long long ne, int Sf, const double* R, const double* Uf,
+                     const double* divFf, double* solc) {
   FusedState& st = fs(c);
-  const long long rows = (long long)c.z.E * c.z.N;
-  Launch L(c, "pmg_restrict");
+  if (e0 < 0 || ne < 0 || e0 + ne > c.z.E) throw std::invalid_argument("restriction rows out of range");
+  if (ne == 0) return;
+  const long long rows = ne * c.z.N;
   const int Sc = c.z.S, nr = 256 / Sc;
+  const long long off = e0 * c.z.N * Sc;  // the first coarse element's rows
+  Launch L(c, "pmg_restrict");
   const size_t lds = ((size_t)Sc * Sf + 2 * (size_t)nr * Sf) * sizeof(double);
   if (nr >= 1 && lds <= kTransferLds)
     hipLaunchKernelGGL(k_restrict_em_lds, dim3((unsigned)((rows + nr - 1) / nr)), dim3(nr * Sc), lds, c.stream,
-                       rows, Sc, Sf, opgemm_chunk(Sf), nr, R, Uf, divFf, st.Ue, st.srce, solc);
+                       rows, Sc, Sf, opgemm_chunk(Sf), nr, R, Uf, divFf, st.Ue + off, st.srce + off, solc + off);
   else
     hipLaunchKernelGGL(k_restrict_em, dim3(nblk(rows * Sc, 256)), dim3(256), 0, c.stream, rows, Sc, Sf,
-                       opgemm_chunk(Sf), R, Uf, divFf, st.Ue, st.srce, solc);
+                       opgemm_chunk(Sf), R, Uf, divFf, st.Ue + off, st.srce + off, solc + off);
   FCHK(hipGetLastError());
 }
 

@@ -181,7 +181,10 @@ void fused_em_residual(Ctx& c, bool mixed, int how = 0);
 void fused_em_set_src(Ctx& c, bool on);
 double* fused_em_array(Ctx& c, int which);
 void fused_em_export_u(Ctx& c);
-void pmg_restrict_em(Ctx& c, int Sf, const double* R, const double* Uf, const double* divFf, double* solc);
+// (restriction into the coarse elements [e0, e0 + ne) from the fine rows Uf,
+// divFf of those elements)
+void pmg_restrict_em(Ctx& c, long long e0, long long ne, int Sf, const double* R, const double* Uf,
+                     const double* divFf, double* solc);
 void pmg_prolong_em(Ctx& up, int Sc, const double* Pm, const double* Uc, const double* solc);
 void fused_residual(Ctx& c, bool mixed);  // divF_spts[0] and wave speeds of the current U
 // block sizes of the fused kernels (extrapolate, face_inviscid, volume, face_boundary, rk)

@@ -252,6 +252,25 @@ int fhip_pmg_create(fhip_ctx* const* levels, int n_levels, const double* const* 
 int fhip_pmg_cycle(fhip_pmg* mg, int path);
 void fhip_pmg_destroy(fhip_pmg* mg);
 
+/* Batched P-multigrid (no counterpart in the reference: it serves callers
+ * that advance many independent meshes of several orders, fhip_update_many):
+ * one cycle for fine contexts of distinct orders, whose coarse levels of one
+ * order are merged across the fine contexts, so that a cycle launches each
+ * coarse order's kernels once for every mesh.  fine[0..n_fine-1]: orders
+ * ascending, each above coarse[0]'s; coarse[j] has order coarse[0]'s + j and
+ * holds, in this order, the meshes of the fine context of order + 1 (if
+ * any) and then those of coarse[j+1]; the top coarse level's order is the
+ * highest fine order - 1.  opp_restrict[j] / opp_prolong[j] map order
+ * coarse[j] + 1 to coarse[j] and back, as for fhip_pmg_create.  Every mesh
+ * goes through the operations of its own fhip_pmg_create cycle, in the same
+ * order, so its result is the same bit for bit.  fhip_pmg_cycle queues the
+ * cycle on fine[0]'s stream after the work queued on every fine context's
+ * stream, and the fine contexts' later work after the cycle.  Fused paths
+ * of one precision only. */
+int fhip_pmg_create_batch(fhip_ctx* const* fine, int n_fine, fhip_ctx* const* coarse, int n_coarse,
+                          const double* const* opp_restrict, const double* const* opp_prolong, int smooth_steps,
+                          fhip_pmg** out);
+
 /* Scan U_spts for NaN/Inf, rho <= 0 or p <= 0; returns FHIP_ESTATE if found. */
 int fhip_check_state(fhip_ctx* ctx);
 

@@ -62,15 +62,18 @@ def test_steady_stop_per_mesh():
     b.close()
 
 
-def test_batch_pmg_equals_alone():
-    """Batched P-multigrid (coarse groups merged like the fine ones) gives each
-    mesh the same state as its own Solver + MultiGrid, bit for bit."""
+@pytest.mark.parametrize("merged", [True, False])
+def test_batch_pmg_equals_alone(merged):
+    """Batched P-multigrid gives each mesh the same state as its own Solver +
+    MultiGrid, bit for bit: with one cycle over coarse levels merged across
+    the orders (fhip_pmg_create_batch), and with a cycle per order group."""
     from flurrypp_amd.solver import MultiGrid
     ms = []
     for nx, ny, P, M in ((8, 4, 2, 0.3), (10, 4, 2, 0.4), (8, 5, 3, 0.25), (6, 4, 1, 0.35)):
         s, U = cases.bump2d(nx=nx, ny=ny, order=P, mach=M, seed=nx * 7 + ny)
         ms.append(Member(s, U, ("bump", nx, ny, P, M)))
-    b = BatchedSolver(ms, path=PATH_FUSED, pmg=True, low_order=0)
+    b = BatchedSolver(ms, path=PATH_FUSED, pmg=True, low_order=0, merged_cycle=merged)
+    assert (b._mgb is not None) == merged
     b.step(6)
     got = b.states()
     for i, m in enumerate(ms):
@@ -97,6 +100,28 @@ def test_batch_pmg_steady_converges():
     print("pmg steady steps", out["steps"], [r[0] for r in out["residual"]])
     assert out["converged"].all(), out
     b.close()
+
+
+@pytest.mark.parametrize("path", [PATH_FUSED, PATH_MIXED])
+def test_batch_merged_cycle_equals_group_cycles(path):
+    """The merged cycle and one cycle per order group give the same states bit
+    for bit, over orders 1..4 with two smoothing steps, a gap in the orders (no
+    P3 group), one-member groups and members frozen by a steady stop."""
+    ms = []
+    for nx, ny, P, M in ((8, 4, 1, 0.3), (10, 5, 2, 0.4), (9, 4, 2, 0.25), (7, 4, 4, 0.35), (6, 5, 4, 0.2)):
+        s, U = cases.bump2d(nx=nx, ny=ny, order=P, mach=M, seed=nx * 13 + ny, time_type=6, cfl=0.9)
+        ms.append(Member(s, U, ("bump", nx, ny, P, M)))
+    out = []
+    for merged in (True, False):
+        b = BatchedSolver(ms, path=path, pmg=True, low_order=0, smooth_steps=2, merged_cycle=merged)
+        r = b.solve_steady(tol=1e-6, max_steps=300, check_every=20)
+        out.append((b.states(), r["steps"], r["converged"]))
+        b.close()
+    (s1, k1, c1), (s2, k2, c2) = out
+    assert list(k1) == list(k2) and list(c1) == list(c2)
+    print("steps", list(k1), "converged", list(c1))
+    for i, (x, y) in enumerate(zip(s1, s2)):
+        assert np.array_equal(x, y), f"member {ms[i].tag} differs (max {np.abs(x - y).max():.3e})"
 
 
 def test_steady_stops_at_max_steps():
