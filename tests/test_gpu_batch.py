@@ -73,7 +73,7 @@ def test_batch_pmg_equals_alone(merged):
         s, U = cases.bump2d(nx=nx, ny=ny, order=P, mach=M, seed=nx * 7 + ny)
         ms.append(Member(s, U, ("bump", nx, ny, P, M)))
     b = BatchedSolver(ms, path=PATH_FUSED, pmg=True, low_order=0, merged_cycle=merged)
-    assert (b._mgb is not None) == merged
+    assert bool(b._merged) == merged
     b.step(6)
     got = b.states()
     for i, m in enumerate(ms):
@@ -104,9 +104,10 @@ def test_batch_pmg_steady_converges():
 
 @pytest.mark.parametrize("path", [PATH_FUSED, PATH_MIXED])
 def test_batch_merged_cycle_equals_group_cycles(path):
-    """The merged cycle and one cycle per order group give the same states bit
-    for bit, over orders 1..4 with two smoothing steps, a gap in the orders (no
-    P3 group), one-member groups and members frozen by a steady stop."""
+    """The merged cycles and one cycle per order group give the same states
+    bit for bit, over orders 1, 2 and 4 with two smoothing steps, a gap in the
+    orders (no P3 group: the P4 cycle's order-2 level holds P4 meshes only)
+    and members frozen by a steady stop."""
     ms = []
     for nx, ny, P, M in ((8, 4, 1, 0.3), (10, 5, 2, 0.4), (9, 4, 2, 0.25), (7, 4, 4, 0.35), (6, 5, 4, 0.2)):
         s, U = cases.bump2d(nx=nx, ny=ny, order=P, mach=M, seed=nx * 13 + ny, time_type=6, cfl=0.9)
@@ -114,6 +115,9 @@ def test_batch_merged_cycle_equals_group_cycles(path):
     out = []
     for merged in (True, False):
         b = BatchedSolver(ms, path=path, pmg=True, low_order=0, smooth_steps=2, merged_cycle=merged)
+        # timeType 6: orders <= 2 and > 2 have their own RK coefficients, so
+        # two merged cycles (P1 and P2; P4)
+        assert [[b.groups[gi].m.order for gi in mc["fine"]] for mc in b._merged] == ([[1, 2], [4]] if merged else [])
         r = b.solve_steady(tol=1e-6, max_steps=300, check_every=20)
         out.append((b.states(), r["steps"], r["converged"]))
         b.close()
