@@ -701,6 +701,9 @@ __device__ __forceinline__ void volume_body(int E, const double* __restrict__ Ue
   R* const sN = sF + EB * N * SF;
   int* const sP = reinterpret_cast<int*>(sN + EB * D * F);
   R* const sD = sN;
+  // viscous runs with kJumpInVolume: this side's LDG part at the block's
+  // flux points, kept from the first face loop for the last one [EB*N*F]
+  R* const sB = reinterpret_cast<R*>(sP + EB * F);
   const int e0 = blockIdx.x * EB, nb = min(EB, E - e0);
   const int b = threadIdx.x / S, s = threadIdx.x - b * S;
   const int sw = spt_lds(D, P1, s);
@@ -890,7 +893,10 @@ __device__ __forceinline__ void volume_body(int E, const double* __restrict__ Ue
         }
         ldg_add_side<D, R>(side == 0, u, g, nv, bv, ph);
 #pragma unroll
-        for (int k = 0; k < N; k++) st_s<NT>(bv[k], &vf.Bve[(ee * N + k) * F + f]);
+        for (int k = 0; k < N; k++) {
+          st_s<NT>(bv[k], &vf.Bve[(ee * N + k) * F + f]);
+          if constexpr (kJumpInVolume<R>) sB[(bb * N + k) * F + f] = bv[k];
+        }
       }
     }
     // under the register hold, the metric terms again (from cache): keeping
@@ -992,9 +998,10 @@ __device__ __forceinline__ void volume_body(int E, const double* __restrict__ Ue
     if constexpr (VISC && kJumpInVolume<R>) {
       side = sP[bb * F + f] & 3;
       dA = vf.dA[ee * F + f];
-      // loaded for every point (all in flight at once); unused at boundary points
+      // this thread's own LDG parts from the first face loop (the same
+      // points); read for every point, unused at boundary points
 #pragma unroll
-      for (int k = 0; k < N; k++) bv[k] = vf.Bve[(ee * N + k) * F + f];
+      for (int k = 0; k < N; k++) bv[k] = sB[(bb * N + k) * F + f];
     }
     if constexpr (!VISC) {
       // the common flux (times dA) the face kernel left at this point
@@ -1843,7 +1850,9 @@ struct Runner {
     const size_t SF = (visc && F > D * S) ? F : D * S;
     if (k == kExtrap) return EB * N * S * sizeof(R);
     if (k == kVolume)
-      return EB * N * (S + SF) * sizeof(R) + (visc ? EB * (D * F * sizeof(R) + F * sizeof(int)) : EB * N * F * sizeof(R));
+      return EB * N * (S + SF) * sizeof(R) +
+             (visc ? EB * (D * F * sizeof(R) + F * sizeof(int) + (kJumpInVolume<R> ? N * F * sizeof(R) : 0))
+                   : EB * N * F * sizeof(R));
     if (k == kRk) return EB * N * (S + F) * sizeof(R);
     return 0;
   }
