@@ -554,6 +554,7 @@ static void pmg_free(fhip_pmg* mg) {
   if (!mg) return;
   if (!mg->lv.empty()) (void)hipSetDevice(mg->owner()->c.device);
   if (!mg->lv.empty()) (void)hipStreamSynchronize(mg->owner()->c.stream);
+  for (fhip_ctx* h : mg->fine) (void)hipStreamSynchronize(h->c.stream);  // their waits on ev[0]
   for (hipGraphExec_t g : mg->graph)
     if (g) (void)hipGraphExecDestroy(g);
   for (hipEvent_t e : mg->ev)
