@@ -391,6 +391,9 @@ template <bool NT, class R>
 __device__ __forceinline__ R ld_face(const R* p) {
   return ld_s<NT && sizeof(R) == 8>(p);
 }
+// A product rounded on its own (never contracted into a following add).
+__device__ __forceinline__ double mul_rn(double a, double b) { return __dmul_rn(a, b); }
+__device__ __forceinline__ float mul_rn(float a, float b) { return __fmul_rn(a, b); }
 
 // ---------------------------------------------------------------------------
 // Common inviscid flux at face flux points; pL/pR are ele*F+fpt.
@@ -447,15 +450,22 @@ __global__ __launch_bounds__(kMaxBlock) void k_face_inv(int nFF, const int* __re
   else {
     if constexpr (D == 2) roe2d<R>(UL, UR, n, fn, R(ph.gamma));
   }
-  // the inviscid common flux times dA on both sides; viscous runs add the
-  // LDG flux in the RK kernel, where both sides' parts meet
-  const R dAL = ld_s<NT>(&dAe[pl]);
+  if (ph.viscous) {
+    // viscous runs: the inviscid common flux once per point pair, at its left
+    // point; the RK kernel scales it by each side's dA (negated on the right)
+    // and adds the LDG flux there, where both sides' parts meet
 #pragma unroll
-  for (int k = 0; k < N; k++) st_s<NT>(fn[k] * dAL, &Fne[bL + k * F]);
-  if (pr >= 0) {
-    const R dAR = ld_s<NT>(&dAe[pr]);
+    for (int k = 0; k < N; k++) st_s<NT>(fn[k], &Fne[bL + k * F]);
+  } else {
+    // inviscid runs: times dA on both sides, for the volume kernel
+    const R dAL = ld_s<NT>(&dAe[pl]);
 #pragma unroll
-    for (int k = 0; k < N; k++) st_s<NT>(-fn[k] * dAR, &Fne[bR + k * F]);
+    for (int k = 0; k < N; k++) st_s<NT>(fn[k] * dAL, &Fne[bL + k * F]);
+    if (pr >= 0) {
+      const R dAR = ld_s<NT>(&dAe[pr]);
+#pragma unroll
+      for (int k = 0; k < N; k++) st_s<NT>(-fn[k] * dAR, &Fne[bR + k * F]);
+    }
   }
   if (ph.viscous) {
     R pf = (pr < 0) ? R(0) : ldg_pen<D, R>(n, R(ph.pen_fact));
@@ -1097,8 +1107,8 @@ __global__ __launch_bounds__(kMaxBlock) void k_face_bnd(int nB, const int* __res
 // Correction of the divergence, RK update and U at the flux points, after the
 // viscous volume kernel.  With NT, the arrays this kernel reads or writes
 // exactly once per launch go through nontemporal loads and stores (they
-// stream past the caches); arrays read twice (the LDG side parts, read by
-// both sides of a face) stay cached.
+// stream past the caches); arrays read twice (the LDG side parts and the
+// inviscid common flux, read by both sides of a face) stay cached.
 template <int D, int P, class R, bool NT>
 __global__ __launch_bounds__(kMaxBlock) void k_rk(int E, const R* __restrict__ divFe, const R* __restrict__ Fne,
                                              const R* __restrict__ disFne, RkArgs<R> ra, Tabs tb,
@@ -1118,8 +1128,8 @@ __global__ __launch_bounds__(kMaxBlock) void k_rk(int E, const R* __restrict__ d
   constexpr int KF = (F + S - 1) / S;
   if constexpr (kJumpInVolume<R>) {
     // viscous: disFne holds this side's jump part +-B dA - disFn; add the
-    // inviscid part (Fne, already times dA) and the partner side's B times dA,
-    // negated on the right side
+    // inviscid part (Fne at the pair's left point, times dA) and the partner
+    // side's B times dA, negated on the right side
     int cd[KF];
 #pragma unroll
     for (int u = 0; u < KF; u++) {
@@ -1142,13 +1152,15 @@ __global__ __launch_bounds__(kMaxBlock) void k_rk(int E, const R* __restrict__ d
 #pragma unroll
       for (int k = 0; k < N; k++) {
         const R bp = Bve[pb + k * F];
-        const R base = ld_s<NT>(&Fne[ob + k * F]) + ld_s<NT>(&disFne[ob + k * F]);
+        // the pair's inviscid flux (at its left point) times this side's dA
+        const R fo = mul_rn(Fne[(side == 1 ? pb : ob) + k * F], dA);
+        const R base = (side == 1 ? -fo : fo) + ld_s<NT>(&disFne[ob + k * F]);
         sD[(bb * N + k) * F + f] = (side == 2) ? base : base + (side == 0 ? bp : -bp) * dA;
       }
     }
   } else {
-    // viscous: common flux = inviscid part (Fne, already times dA) plus both
-    // sides' LDG parts times dA, negated on the right side
+    // viscous: common flux = inviscid part (Fne at the pair's left point)
+    // plus both sides' LDG parts, times dA, negated on the right side
     int cd[KF];
 #pragma unroll
     for (int u = 0; u < KF; u++) {
@@ -1173,8 +1185,9 @@ __global__ __launch_bounds__(kMaxBlock) void k_rk(int E, const R* __restrict__ d
         const R bo = Bve[ob + k * F], bp = Bve[pb + k * F];
         R bv = (side != 2) ? bo + bp : bo;
         if (side == 1) bv = -bv;
-        sD[(bb * N + k) * F + f] = (ld_s<NT>(&Fne[ob + k * F]) + bv * dA) -
-                                   ld_s<NT>(&disFne[ob + k * F]);
+        // the pair's inviscid flux (at its left point) times this side's dA
+        const R fo = mul_rn(Fne[(side == 1 ? pb : ob) + k * F], dA);
+        sD[(bb * N + k) * F + f] = ((side == 1 ? -fo : fo) + bv * dA) - ld_s<NT>(&disFne[ob + k * F]);
       }
     }
   }

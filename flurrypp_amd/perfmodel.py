@@ -27,12 +27,15 @@ def fused_work(D: int, P: int, E: int, nFF: int, nFFb: int, viscous: bool, nRK: 
     F = 2 * D * P1 ** (D - 1)
     nFFi = nFF - nFFb
     out = {}
-    # face_inviscid: read UL, UR (interior), normal, dA L (+R); write Fn*dA L
-    # (+R), wave speed (+ dUc L (+R) in viscous runs); 3 int32 per face flux point
-    rd = nFF * (N + D + 1 + 3 * B4 / B8) + nFFi * (N + 1)
-    wr = nFF * (N + 1) + nFFi * N
+    # face_inviscid: read UL, UR (interior), normal, 3 int32 per face flux
+    # point; inviscid runs: dA L (+R) in, Fn*dA L (+R) out; viscous runs: Fn
+    # once per pair (at L) and dUc L (+R) out; the wave speed
     if viscous:
-        wr += nFF * N + nFFi * N
+        rd = nFF * (N + D + 3 * B4 / B8) + nFFi * N
+        wr = nFF * (N + 1) + nFF * N + nFFi * N
+    else:
+        rd = nFF * (N + D + 1 + 3 * B4 / B8) + nFFi * (N + 1)
+        wr = nFF * (N + 1) + nFFi * N
     out["fused_face_inviscid"] = Work((rd + wr) * B8, 0.0)
     # volume
     rd = E * (N * S + D * D * S)
@@ -98,12 +101,14 @@ def mixed_work(D: int, P: int, E: int, nFF: int, nFFb: int, viscous: bool, nRK: 
     f4, d8, i4 = 4, 8, 4
     fp = fused_work(D, P, E, nFF, nFFb, viscous, nRK, time_type)
     out = {}
-    # face: 3 ints, UL, UR, normal, dA L (+R) in; Fn*dA L (+R), wave speed
-    # (FP64) [+ dUc L (+R)] out
-    rd = nFF * (3 * i4 + f4 * (N + D + 1)) + nFFi * f4 * (N + 1)
-    wr = nFF * (f4 * N + d8) + nFFi * f4 * N
+    # face: 3 ints, UL, UR, normal in [+ dA L (+R) inviscid]; wave speed
+    # (FP64) out, Fn*dA L (+R) inviscid, Fn once per pair and dUc L (+R) viscous
     if viscous:
-        wr += (nFF + nFFi) * f4 * N
+        rd = nFF * (3 * i4 + f4 * (N + D)) + nFFi * f4 * N
+        wr = nFF * (f4 * N + d8) + (nFF + nFFi) * f4 * N
+    else:
+        rd = nFF * (3 * i4 + f4 * (N + D + 1)) + nFFi * f4 * (N + 1)
+        wr = nFF * (f4 * N + d8) + nFFi * f4 * N
     out["mixed_face_inviscid"] = Work(rd + wr, 0.0)
     # volume: U (stage copy), metric terms
     rd = E * f4 * (N * S + D * D * S)
