@@ -419,7 +419,7 @@ __device__ __forceinline__ void bc_state(int code, const R* UL, const R* n, R* U
 template <int D, int P, class R, bool NT>
 __global__ __launch_bounds__(kMaxBlock) void k_face_inv(int nFF, const int* __restrict__ pLa, const int* __restrict__ pRa,
                                                    const int* __restrict__ bca, const R* __restrict__ Ufe,
-                                                   const R* __restrict__ norme, const R* __restrict__ dAe,
+                                                   const R* __restrict__ norme,
                                                    R* __restrict__ Fne, double* __restrict__ waveSpe,
                                                    R* __restrict__ dUce, Phys ph,
                                                    const double* __restrict__ bst) {
@@ -450,23 +450,12 @@ __global__ __launch_bounds__(kMaxBlock) void k_face_inv(int nFF, const int* __re
   else {
     if constexpr (D == 2) roe2d<R>(UL, UR, n, fn, R(ph.gamma));
   }
-  if (ph.viscous) {
-    // viscous runs: the inviscid common flux once per point pair, at its left
-    // point; the RK kernel scales it by each side's dA (negated on the right)
-    // and adds the LDG flux there, where both sides' parts meet
+  // the common inviscid flux once per point pair, at its left point; the
+  // kernel that applies the correction scales it by each side's dA (negated
+  // on the right), and viscous runs add the LDG flux there, where both
+  // sides' parts meet
 #pragma unroll
-    for (int k = 0; k < N; k++) st_s<NT>(fn[k], &Fne[bL + k * F]);
-  } else {
-    // inviscid runs: times dA on both sides, for the volume kernel
-    const R dAL = ld_s<NT>(&dAe[pl]);
-#pragma unroll
-    for (int k = 0; k < N; k++) st_s<NT>(fn[k] * dAL, &Fne[bL + k * F]);
-    if (pr >= 0) {
-      const R dAR = ld_s<NT>(&dAe[pr]);
-#pragma unroll
-      for (int k = 0; k < N; k++) st_s<NT>(-fn[k] * dAR, &Fne[bR + k * F]);
-    }
-  }
+  for (int k = 0; k < N; k++) st_s<NT>(fn[k], &Fne[bL + k * F]);
   if (ph.viscous) {
     R pf = (pr < 0) ? R(0) : ldg_pen<D, R>(n, R(ph.pen_fact));
 #pragma unroll
@@ -701,16 +690,16 @@ __device__ __forceinline__ void volume_body(int E, const double* __restrict__ Ue
   // sF holds, in turn (viscous runs): dUc at the flux points, the physical
   // gradient until it has been extrapolated to the faces, then the transformed
   // flux at the same indices as the gradient.  [EB*N*S] + [EB*N*SF]; viscous
-  // runs also stage the faces' normals and the points' side codes for the
-  // face loop, [EB*D*F] + [EB*F] ints, inviscid runs keep the flux jump of
-  // the RK update there, [EB*N*F]
+  // runs also stage the faces' normals for the face loop, [EB*D*F], inviscid
+  // runs keep the flux jump of the RK update there, [EB*N*F]; then the
+  // points' side codes [EB*F] ints
   constexpr int SF = (VISC && F > D * S) ? F : D * S;
   R* const sU = reinterpret_cast<R*>(fhip_lds);
   R* const sF = sU + EB * N * S;
   R* const sG = sF;
   R* const sN = sF + EB * N * SF;
-  int* const sP = reinterpret_cast<int*>(sN + EB * D * F);
   R* const sD = sN;
+  int* const sP = reinterpret_cast<int*>(VISC ? sN + EB * D * F : sD + EB * N * F);
   // viscous runs with kJumpInVolume: this side's LDG part at the block's
   // flux points, kept from the first face loop for the last one [EB*N*F]
   R* const sB = reinterpret_cast<R*>(sP + EB * F);
@@ -748,7 +737,7 @@ __device__ __forceinline__ void volume_body(int E, const double* __restrict__ Ue
     // their latencies overlap: a thread stages at most N values of U (the
     // block holds at most blockDim / S elements) and KC of dUc
     constexpr int KC = VISC ? (N * F + S - 1) / S : 1, KN = VISC ? (D * F + S - 1) / S : 1,
-                  KP = VISC ? (F + S - 1) / S : 1;
+                  KP = (F + S - 1) / S;
     const int nU = nb * N * S, nC = nb * N * F, nN = nb * D * F, nP = nb * F;
     R vu[N], vc[KC], vn[KN];
     int vp[KP];
@@ -769,11 +758,11 @@ __device__ __forceinline__ void volume_body(int E, const double* __restrict__ Ue
         const int i = threadIdx.x + u * blockDim.x;
         if (i < nN) vn[u] = ld_s<NT>(&vf.nfe[(long long)e0 * D * F + i]);
       }
+    }
 #pragma unroll
-      for (int u = 0; u < KP; u++) {
-        const int i = threadIdx.x + u * blockDim.x;
-        if (i < nP) vp[u] = vf.fpart[(long long)e0 * F + i];
-      }
+    for (int u = 0; u < KP; u++) {
+      const int i = threadIdx.x + u * blockDim.x;
+      if (i < nP) vp[u] = vf.fpart[(long long)e0 * F + i];
     }
 #pragma unroll
     for (int u = 0; u < N; u++) {
@@ -794,14 +783,43 @@ __device__ __forceinline__ void volume_body(int E, const double* __restrict__ Ue
         const int i = threadIdx.x + u * blockDim.x;
         if (i < nN) sN[i] = vn[u];
       }
+    }
 #pragma unroll
-      for (int u = 0; u < KP; u++) {
-        const int i = threadIdx.x + u * blockDim.x;
-        if (i < nP) sP[i] = vp[u];
-      }
+    for (int u = 0; u < KP; u++) {
+      const int i = threadIdx.x + u * blockDim.x;
+      if (i < nP) sP[i] = vp[u];
     }
   }
   __syncthreads();
+  // inviscid runs: the common flux at the points this thread finishes in the
+  // last face loop (the pair's left point, the partner's on the right side)
+  // and their dA, loaded now so that the loads are in flight during the flux
+  // and divergence phases
+  constexpr int KFP = VISC ? 1 : (F + S - 1) / S;
+  R pf[KFP][N], pdA[KFP];
+  int psd[KFP];
+  if constexpr (!VISC) {
+#pragma unroll
+    for (int u = 0; u < KFP; u++) {
+      const int i = threadIdx.x + u * blockDim.x;
+      psd[u] = 2;
+      pdA[u] = 0;
+#pragma unroll
+      for (int k = 0; k < N; k++) pf[u][k] = 0;
+      if (i < nb * F) {
+        const int code = sP[i], bb = i / F;
+        psd[u] = code & 3;
+        long long pb = (long long)(e0 + bb) * N * F + (i - bb * F);
+        if (psd[u] == 1) {
+          const long long p = code >> 2, ep = p / F;
+          pb = ep * N * F + (p - ep * F);
+        }
+#pragma unroll
+        for (int k = 0; k < N; k++) pf[u][k] = Fne[pb + k * F];
+        pdA[u] = vf.dA[(long long)e0 * F + i];
+      }
+    }
+  }
   if constexpr (VISC) {
     // phase 1: reference-space gradient (opp_grad_spts) plus correction
     // (opp_correctU), then the physical gradient (correctGradU transform,
@@ -995,37 +1013,54 @@ __device__ __forceinline__ void volume_body(int E, const double* __restrict__ Ue
   // +-B dA - disFn (B as written by this thread in the face loop above, - on
   // the right side; boundary points get - disFn here and + B dA from the
   // boundary kernel).
-  for (int i = threadIdx.x; i < nb * F; i += blockDim.x) {
-    const int bb = i / F, f = i - bb * F;
-    const long long ee = e0 + bb;
-    const int* xi = tb.ext_idx + f * P1;
-    // extrapolateNormalFlux: of the sum over directions only the face's own
-    // normal direction is non-zero (the other terms are exact zeros)
-    const int dn = tb.efn_dir[f];
-    const R* ew = tw<R>(tb.efn_w, tb.efn_wf) + f * P1;
-    int side = 2;
-    R dA = 0, bv[N];
-    if constexpr (VISC && kJumpInVolume<R>) {
-      side = sP[bb * F + f] & 3;
-      dA = vf.dA[ee * F + f];
-      // this thread's own LDG parts from the first face loop (the same
-      // points); read for every point, unused at boundary points
+  if constexpr (VISC) {
+    for (int i = threadIdx.x; i < nb * F; i += blockDim.x) {
+      const int bb = i / F, f = i - bb * F;
+      const long long ee = e0 + bb;
+      const int* xi = tb.ext_idx + f * P1;
+      // extrapolateNormalFlux: of the sum over directions only the face's own
+      // normal direction is non-zero (the other terms are exact zeros)
+      const int dn = tb.efn_dir[f];
+      const R* ew = tw<R>(tb.efn_w, tb.efn_wf) + f * P1;
+      int side = 2;
+      R dA = 0, bv[N];
+      if constexpr (kJumpInVolume<R>) {
+        side = sP[bb * F + f] & 3;
+        dA = vf.dA[ee * F + f];
+        // this thread's own LDG parts from the first face loop (the same
+        // points); read for every point, unused at boundary points
 #pragma unroll
-      for (int k = 0; k < N; k++) bv[k] = sB[(bb * N + k) * F + f];
+        for (int k = 0; k < N; k++) bv[k] = sB[(bb * N + k) * F + f];
+      }
+#pragma unroll
+      for (int k = 0; k < N; k++) {
+        R a = 0;
+#pragma unroll
+        for (int m = 0; m < P1; m++) a += ew[m] * sF[((bb * D + dn) * N + k) * S + xi[m]];
+        if constexpr (kJumpInVolume<R>) a = (side == 2) ? -a : (side == 0 ? bv[k] : -bv[k]) * dA - a;
+        st_s<NT>(a, &disFne[(ee * N + k) * F + f]);
+      }
     }
-    if constexpr (!VISC) {
-      // the common flux (times dA) the face kernel left at this point
+  } else {
+    // inviscid runs: the flux jump, the common flux (loaded above) times
+    // this side's dA, negated on the right side, less the extrapolated flux
 #pragma unroll
-      for (int k = 0; k < N; k++) bv[k] = ld_s<NT>(&Fne[(ee * N + k) * F + f]);
-    }
+    for (int u = 0; u < KFP; u++) {
+      const int i = threadIdx.x + u * blockDim.x;
+      if (i < nb * F) {
+        const int bb = i / F, f = i - bb * F;
+        const int* xi = tb.ext_idx + f * P1;
+        const int dn = tb.efn_dir[f];
+        const R* ew = tw<R>(tb.efn_w, tb.efn_wf) + f * P1;
 #pragma unroll
-    for (int k = 0; k < N; k++) {
-      R a = 0;
+        for (int k = 0; k < N; k++) {
+          R a = 0;
 #pragma unroll
-      for (int m = 0; m < P1; m++) a += ew[m] * sF[((bb * D + dn) * N + k) * S + xi[m]];
-      if constexpr (VISC && kJumpInVolume<R>) a = (side == 2) ? -a : (side == 0 ? bv[k] : -bv[k]) * dA - a;
-      if constexpr (VISC) st_s<NT>(a, &disFne[(ee * N + k) * F + f]);
-      else sD[(bb * N + k) * F + f] = bv[k] - a;
+          for (int m = 0; m < P1; m++) a += ew[m] * sF[((bb * D + dn) * N + k) * S + xi[m]];
+          const R fo = mul_rn(pf[u][k], pdA[u]);
+          sD[(bb * N + k) * F + f] = (psd[u] == 1 ? -fo : fo) - a;
+        }
+      }
     }
   }
   if constexpr (!VISC) {
@@ -1754,13 +1789,17 @@ bool fused_setup(Ctx& c, const fhip_desc* d) {
     FCHK(hipMemcpy(st->pL, pl.data(), pl.size() * 4, hipMemcpyHostToDevice));
     FCHK(hipMemcpy(st->pR, pr.data(), pr.size() * 4, hipMemcpyHostToDevice));
     FCHK(hipMemcpy(st->bc, bc.data(), bc.size() * 4, hipMemcpyHostToDevice));
-    if (c.prm.viscous) {
-      // every element flux point must lie on exactly one face point pair
+    {
+      // every element flux point must lie on exactly one face point pair;
+      // its code: 4 * partner flux point + side, or 4 * boundary slot + 2
+      // (viscous runs also keep the face's normal at every point)
+      const bool visc = c.prm.viscous != 0;
       if (E * F >= (1LL << 29)) throw std::invalid_argument("fused path: more than 2^29 element flux points");
       std::vector<int> part((size_t)E * F, -1), bt;
-      std::vector<double> nl((size_t)E * D * F);
+      std::vector<double> nl(visc ? (size_t)E * D * F : 0);
       // the left element's normal at flux point q (element-major q = e*F + f)
       auto put_normal = [&](int q, int qL) {
+        if (!visc) return;
         const long long e = q / F, f = q - e * F, eL = qL / F, fL = qL - eL * F;
         for (long long x = 0; x < D; x++) nl[(e * D + x) * F + f] = d->norm_fpts[(fL * E + eL) * D + x];
       };
@@ -1784,13 +1823,15 @@ bool fused_setup(Ctx& c, const fhip_desc* d) {
           fused_release(c);
           return false;
         }
-      st->nBnd = (int)bt.size();
       st->fpart = dal<int>(part.size());
-      st->bndT = dal<int>(bt.size());
-      st->nfe = dal<double>(nl.size());
       FCHK(hipMemcpy(st->fpart, part.data(), part.size() * 4, hipMemcpyHostToDevice));
-      FCHK(hipMemcpy(st->nfe, nl.data(), nl.size() * 8, hipMemcpyHostToDevice));
-      if (!bt.empty()) FCHK(hipMemcpy(st->bndT, bt.data(), bt.size() * 4, hipMemcpyHostToDevice));
+      if (visc) {
+        st->nBnd = (int)bt.size();
+        st->bndT = dal<int>(bt.size());
+        st->nfe = dal<double>(nl.size());
+        FCHK(hipMemcpy(st->nfe, nl.data(), nl.size() * 8, hipMemcpyHostToDevice));
+        if (!bt.empty()) FCHK(hipMemcpy(st->bndT, bt.data(), bt.size() * 4, hipMemcpyHostToDevice));
+      }
     }
   }
   return true;
@@ -1865,7 +1906,7 @@ struct Runner {
     if (k == kVolume)
       return EB * N * (S + SF) * sizeof(R) +
              (visc ? EB * (D * F * sizeof(R) + F * sizeof(int) + (kJumpInVolume<R> ? N * F * sizeof(R) : 0))
-                   : EB * N * F * sizeof(R));
+                   : EB * (N * F * sizeof(R) + F * sizeof(int)));
     if (k == kRk) return EB * N * (S + F) * sizeof(R);
     return 0;
   }
@@ -1888,10 +1929,10 @@ struct Runner {
     if (nFF == 0) return;
     if (st.stream && kFaceRowsWhole)
       hipLaunchKernelGGL((k_face_inv<D, P, R, true>), dim3(nblk(nFF, nt)), dim3(nt), 0, c.stream, nFF, st.pL, st.pR,
-                         st.bc, w.Ufe, w.nrm, w.dA, w.Fne, st.waveSpe, w.dUce, c.ph, st.bst);
+                         st.bc, w.Ufe, w.nrm, w.Fne, st.waveSpe, w.dUce, c.ph, st.bst);
     else
       hipLaunchKernelGGL((k_face_inv<D, P, R, false>), dim3(nblk(nFF, nt)), dim3(nt), 0, c.stream, nFF, st.pL, st.pR,
-                         st.bc, w.Ufe, w.nrm, w.dA, w.Fne, st.waveSpe, w.dUce, c.ph, st.bst);
+                         st.bc, w.Ufe, w.nrm, w.Fne, st.waveSpe, w.dUce, c.ph, st.bst);
     FCHK(hipGetLastError());
   }
   template <bool VISC, bool NT>

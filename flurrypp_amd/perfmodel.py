@@ -28,14 +28,12 @@ def fused_work(D: int, P: int, E: int, nFF: int, nFFb: int, viscous: bool, nRK: 
     nFFi = nFF - nFFb
     out = {}
     # face_inviscid: read UL, UR (interior), normal, 3 int32 per face flux
-    # point; inviscid runs: dA L (+R) in, Fn*dA L (+R) out; viscous runs: Fn
-    # once per pair (at L) and dUc L (+R) out; the wave speed
+    # point; write Fn once per pair (at L), the wave speed (+ dUc L (+R) in
+    # viscous runs)
+    rd = nFF * (N + D + 3 * B4 / B8) + nFFi * N
+    wr = nFF * (N + 1)
     if viscous:
-        rd = nFF * (N + D + 3 * B4 / B8) + nFFi * N
-        wr = nFF * (N + 1) + nFF * N + nFFi * N
-    else:
-        rd = nFF * (N + D + 1 + 3 * B4 / B8) + nFFi * (N + 1)
-        wr = nFF * (N + 1) + nFFi * N
+        wr += nFF * N + nFFi * N
     out["fused_face_inviscid"] = Work((rd + wr) * B8, 0.0)
     # volume
     rd = E * (N * S + D * D * S)
@@ -50,8 +48,8 @@ def fused_work(D: int, P: int, E: int, nFF: int, nFFb: int, viscous: bool, nRK: 
     else:
         # inviscid runs: the volume kernel also applies the RK update (below);
         # divF and the discontinuous normal flux stay on chip, the common flux
-        # Fn is read instead
-        vol_rd = rd + E * N * F
+        # Fn is read instead (with dA and the side code)
+        vol_rd = rd + E * N * F + E * F * (1 + B4 / B8)
         vol_wr = wr - E * (N * S + N * F)
     if viscous and nFFb:
         # boundary LDG flux: U, normal, gradient in; the point's flux part out
@@ -101,17 +99,16 @@ def mixed_work(D: int, P: int, E: int, nFF: int, nFFb: int, viscous: bool, nRK: 
     f4, d8, i4 = 4, 8, 4
     fp = fused_work(D, P, E, nFF, nFFb, viscous, nRK, time_type)
     out = {}
-    # face: 3 ints, UL, UR, normal in [+ dA L (+R) inviscid]; wave speed
-    # (FP64) out, Fn*dA L (+R) inviscid, Fn once per pair and dUc L (+R) viscous
+    # face: 3 ints, UL, UR, normal in; Fn once per pair (at L), wave speed
+    # (FP64) [+ dUc L (+R)] out
+    rd = nFF * (3 * i4 + f4 * (N + D)) + nFFi * f4 * N
+    wr = nFF * (f4 * N + d8)
     if viscous:
-        rd = nFF * (3 * i4 + f4 * (N + D)) + nFFi * f4 * N
-        wr = nFF * (f4 * N + d8) + (nFF + nFFi) * f4 * N
-    else:
-        rd = nFF * (3 * i4 + f4 * (N + D + 1)) + nFFi * f4 * (N + 1)
-        wr = nFF * (f4 * N + d8) + nFFi * f4 * N
+        wr += (nFF + nFFi) * f4 * N
     out["mixed_face_inviscid"] = Work(rd + wr, 0.0)
-    # volume: U (stage copy), metric terms
-    rd = E * f4 * (N * S + D * D * S)
+    # volume: U (stage copy), metric terms (inviscid runs: with dA and the
+    # side codes for the common flux)
+    rd = E * f4 * (N * S + D * D * S) + (0 if viscous else E * F * (f4 + i4))
     if viscous:
         # 1/detJ, dUc, face normal, dA, side code; divF, this side's flux jump
         # (with its LDG part), the LDG part, boundary gradient out
