@@ -202,15 +202,16 @@ template <>
 __device__ __forceinline__ const float* tw<float>(const double*, const float* f) { return f; }
 
 // Position of solution point s = i + P1*(j + P1*k) in an element's LDS slab.
-// Each (j,k) row is rotated by j+k and each k plane by k, so that the P1^2
-// points a face's flux points read at one depth fall into distinct LDS banks
-// for all three face orientations (row-major order would give 4-way bank
-// conflicts on the x and y faces at P=3).  A bijection on every plane.
+// Hexes with an even P1: each (j,k) row is rotated by j+k and each k plane by
+// k, so that the P1^2 points a face's flux points read at one depth fall into
+// distinct LDS banks for all three face orientations (row-major order would
+// give 4-way bank conflicts on the x and y faces at P=3).  With an odd P1 (rows
+// of an odd number of doubles) and on quads, row-major order spreads the
+// slab's reads over the banks at least as well (the rotation measured 48% of
+// the P4 volume kernel's LDS cycles as bank conflicts).  A bijection on every
+// plane.
 __host__ __device__ constexpr int spt_lds(int D, int P1, int s) {
-  if (D == 2) {
-    const int i = s % P1, j = s / P1;
-    return j * P1 + (i + j) % P1;
-  }
+  if (D == 2 || P1 % 2 == 1) return s;
   const int i = s % P1, j = (s / P1) % P1, k = s / (P1 * P1);
   return (k * P1 + (j + k) % P1) * P1 + (i + k) % P1;
 }
