@@ -249,9 +249,9 @@ struct FusedState {
   double *Ue = nullptr, *U0e = nullptr, *Uacc = nullptr, *divF0e = nullptr, *JGe = nullptr, *detJe = nullptr,
          *iJe = nullptr, *norme = nullptr, *dAe = nullptr, *waveSpe = nullptr;
   int *pL = nullptr, *pR = nullptr, *bc = nullptr;
-  // viscous runs: per element flux point, 4 * (partner flux point e*F + f) +
-  // side (0 left, 1 right) or 4 * boundary slot + 2; the face's normal (the
-  // left element's) at every element flux point; boundary face points by slot
+  // per element flux point, 4 * (partner flux point e*F + f) + side (0 left,
+  // 1 right) or 4 * boundary slot + 2; viscous runs: the face's normal (the
+  // left element's) at every element flux point, boundary face points by slot
   int *fpart = nullptr, *bndT = nullptr;
   double* nfe = nullptr;
   int nBnd = 0;
@@ -663,8 +663,10 @@ constexpr bool kJumpInVolume = sizeof(R) == 4;
 template <int D, class R>
 constexpr bool kHold = D == 2 || sizeof(R) == 4;
 
-// Viscous runs: what the volume kernel leaves at the flux points in place of
-// the gradient there (see the face loop of volume_body).
+// The volume kernel's flux-point inputs and outputs besides the slabs: the
+// side codes and dA (every run), and for viscous runs what it leaves at the
+// flux points in place of the gradient there (see the face loop of
+// volume_body).
 template <class R>
 struct VFace {
   const int* fpart;    // [E*F] 4 * partner flux point + side, or 4 * boundary slot + 2
