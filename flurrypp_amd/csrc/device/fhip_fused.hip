@@ -577,8 +577,11 @@ __device__ __forceinline__ void rk_tail_p(int E, const R* dv, R* sU, const R* sD
       for (int k = 0; k < N; k++) uin[k] = ld_s<NT>(&Ue[(e * N + k) * S + s]);
     }
     if (needU0) {
+      // with an FP32 stage copy the FP64 state stays the step's U0 until the
+      // last stage writes it, and no separate U0 is kept (U0e null)
+      const double* u0 = U0e ? U0e : Ue;
 #pragma unroll
-      for (int k = 0; k < N; k++) u0in[k] = ld_s<NT>(&U0e[(e * N + k) * S + s]);
+      for (int k = 0; k < N; k++) u0in[k] = ld_s<NT>(&u0[(e * N + k) * S + s]);
     }
     if (needAcc) {
 #pragma unroll
@@ -597,7 +600,7 @@ __device__ __forceinline__ void rk_tail_p(int E, const R* dv, R* sU, const R* sD
       const double u = uin[k];
       double un;
       if (mode == 0) {
-        st_s<NT>(u, &U0e[iu]);
+        if (U0e) st_s<NT>(u, &U0e[iu]);
         if (Rst) st_s<NT>(R(r), &Rst[iu]);
         else st_s<NT>(u - bcoef * r / dj * dt, &Uacc[iu]);
         un = u - a * r / dj * dt;
@@ -617,7 +620,7 @@ __device__ __forceinline__ void rk_tail_p(int E, const R* dv, R* sU, const R* sD
       } else if (mode == 3) {
         un = u - bcoef * r / dj * dt;
       } else if (mode == 6) {
-        st_s<NT>(u, &U0e[iu]);
+        if (U0e) st_s<NT>(u, &U0e[iu]);
         un = u - a * r / dj * dt;
       } else {
         un = u0in[k] - a * r / dj * dt;
@@ -1989,7 +1992,8 @@ struct Runner {
     double* d0 = mode == 7 ? st.srce : (write0 ? st.divF0e : nullptr);
     const double* src = mode == 8 ? st.srce : (st.use_src && mode != 7 ? st.srce : nullptr);
     return RkArgs<R>{mode, a, b, st.detJe, c.d_dt, c.eleDt,
-                     c.prm.dt_type == 2 ? 1 : (c.prm.dt_type == 1 && st.nMesh > 1 ? 2 : 0), st.Ue, st.U0e, st.Uacc,
+                     c.prm.dt_type == 2 ? 1 : (c.prm.dt_type == 1 && st.nMesh > 1 ? 2 : 0), st.Ue,
+                     w.Us ? nullptr : st.U0e, st.Uacc,
                      d0, st.eleMesh, st.meshDt, st.active, src,
                      (w.Rst && !st.use_src) ? w.Rst : nullptr, stage, bs, w.Us,
                      (mode == 2 || mode == 3 || (mode == 4 && stage == c.z.nRK - 1)) ? 1 : 0, w.Ufe};

@@ -119,7 +119,8 @@ def mixed_work(D: int, P: int, E: int, nFF: int, nFFb: int, viscous: bool, nRK: 
             # U, normal, gradient, 3 ints, dA in; the point's flux jump updated
             out["mixed_face_boundary"] = Work(nFFb * (f4 * (N + D + D * N + 1) + 3 * i4 + 2 * f4 * N), 0.0)
     # classical RK stages keep their residuals in FP32 (the form without a
-    # multigrid source); Jameson-form stages read U0 and write the state last
+    # multigrid source); the FP64 state, written at the last stage only, is
+    # also the step's U0; Jameson-form stages read it and write the state last
     per_stage = []
     for st in range(nRK):
         rd_ = E * (f4 * N * S + d8 * S)                # divF, detJ
@@ -136,8 +137,8 @@ def mixed_work(D: int, P: int, E: int, nFF: int, nFFb: int, viscous: bool, nRK: 
             rd_ += E * d8 * N * S
             wr_ += E * d8 * N * S
         elif st == 0:
-            rd_ += E * d8 * N * S                      # U
-            wr_ += E * (d8 * N * S + f4 * N * S)       # U0, residual of stage 0
+            rd_ += E * d8 * N * S                      # U (the step's U0 until the last stage writes it)
+            wr_ += E * f4 * N * S                      # residual of stage 0
         elif st == nRK - 1:
             rd_ += E * (d8 * N * S + f4 * N * S * (nRK - 1))  # U0, stored residuals
             wr_ += E * d8 * N * S                      # U
