@@ -128,6 +128,58 @@ def test_batch_merged_cycle_equals_group_cycles(path):
         assert np.array_equal(x, y), f"member {ms[i].tag} differs (max {np.abs(x - y).max():.3e})"
 
 
+def test_batch_multigrid_rejects_mismatched_levels():
+    """fhip_pmg_create_batch checks that every coarse level holds the meshes of
+    the fine level one order up and of the coarse level above it, and that the
+    fine orders ascend; a refused hierarchy leaves no object behind."""
+    import ctypes as C
+    from flurrypp_amd import _native as nat
+    from flurrypp_amd.batch import BatchGroup, MergedSetup
+    from flurrypp_amd.mesh import pmg_operators
+    from flurrypp_amd.solver import HostSetup
+    import copy
+    ms = []
+    for nx, ny, P in ((8, 4, 1), (6, 4, 2)):
+        s, U = cases.bump2d(nx=nx, ny=ny, order=P, mach=0.3, seed=nx)
+        ms.append(Member(s, U))
+    def level(members, q):
+        cm = []
+        for m in members:
+            pc = copy.copy(m.setup.params)
+            pc.order = q
+            hs = HostSetup(pc, m.setup.mesh)
+            cm.append(Member(hs, hs.initial_condition()))
+        g = BatchGroup(MergedSetup(cm))
+        g.initialize()
+        return g
+    f1, f2 = BatchGroup(MergedSetup([ms[0]])), BatchGroup(MergedSetup([ms[1]]))
+    f1.initialize()
+    f2.initialize()
+    good = [level(ms, 0), level(ms[1:], 1)]     # order 0: both meshes; order 1: the P2 mesh
+    bad = [level(ms[1:], 0), level(ms[1:], 1)]  # order 0 misses the P1 mesh
+    R = [pmg_operators(2, q + 1, ms[0].setup.params.sptsTypeQuad)[1] for q in (0, 1)]
+    Pr = [pmg_operators(2, q, ms[0].setup.params.sptsTypeQuad)[0] for q in (0, 1)]
+    Ra = (nat._dp * 2)(*[nat.dptr(a) for a in R])
+    Pa = (nat._dp * 2)(*[nat.dptr(a) for a in Pr])
+    lib = nat.hip_lib()
+    def create(fine, coarse):
+        h = C.c_void_p()
+        fa = (C.c_void_p * len(fine))(*[g._ctx.value for g in fine])
+        ca = (C.c_void_p * len(coarse))(*[g._ctx.value for g in coarse])
+        rc = lib.fhip_pmg_create_batch(fa, len(fine), ca, len(coarse), Ra, Pa, 1, C.byref(h))
+        return rc, h
+    rc, h = create([f1, f2], bad)
+    assert rc != 0 and not h.value and b"must hold the meshes" in lib.fhip_last_error()
+    rc, h = create([f2, f1], good)
+    assert rc != 0 and not h.value and b"ascend" in lib.fhip_last_error()
+    rc, h = create([f1, f2], good)
+    assert rc == 0 and h.value
+    nat.hip_check(lib.fhip_pmg_cycle(h, PATH_FUSED))
+    lib.fhip_pmg_destroy(h)
+    for g in good + bad + [f1, f2]:
+        g.close()
+
+
 def test_steady_stops_at_max_steps():
     # max_steps not a multiple of check_every: the last chunk is shortened
     ms = members()[:3]
