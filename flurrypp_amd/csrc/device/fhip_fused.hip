@@ -260,6 +260,7 @@ struct FusedState {
   std::vector<int> h_meshStart;
   double *bst = nullptr, *meshDt = nullptr, *meshRes = nullptr;
   unsigned* dtTicket = nullptr;  // blocks of the dt kernel that have finished (zero between launches)
+  double* vole = nullptr;        // element volumes, sum_s detJ w_s (calcDt's numerator)
   int *eleMesh = nullptr, *meshStart = nullptr, *active = nullptr;
   // P-multigrid source term (element-major copy of src_spts), used while use_src
   double* srce = nullptr;
@@ -277,7 +278,7 @@ struct FusedState {
     w32.release();
     void* ps[] = {tab_mem, Ue,  U0e,     Uacc,    divF0e,    JGe,    detJe,  iJe,   norme, dAe, waveSpe, pL,
                   pR,      bc,  fpart,   bndT,    nfe,       bst,    meshDt, meshRes, eleMesh, meshStart, active, srce,
-                  dtTicket};
+                  dtTicket, vole};
     for (void* p : ps)
       if (p) (void)hipFree(p);
   }
@@ -1413,32 +1414,48 @@ __device__ __forceinline__ double block_min(double v, double* red) {
   return v;
 }
 
+// The element volumes calcDt divides by, once per context (the geometry is
+// fixed): the same sum as the reference's per call.
+template <int D, int P>
+__global__ __launch_bounds__(256) void k_ele_vol(int E, const double* __restrict__ detJe, const double* __restrict__ ws,
+                                                  double* __restrict__ vole) {
+  constexpr int S = Shape<D, P>::S;
+  const int e = blockIdx.x * blockDim.x + threadIdx.x;
+  if (e >= E) return;
+  double vol = 0;
+  for (int s = 0; s < S; s++) vol += detJe[(long long)e * S + s] * ws[s];
+  vole[e] = vol;
+}
+
 // ele::calcDt per element (element-major inputs) and block minima.  With a
 // ticket counter, the block that finishes last also reduces the block minima
 // to the global dt (solver::calcDt's minimum, src/solver.cpp:447-468) and
-// resets the counter to zero for the next launch.
+// resets the counter to zero for the next launch (a single block's minimum is
+// the global dt at once).
 template <int D, int P>
-__global__ __launch_bounds__(256) void k_ele_dt_em(int E, const double* __restrict__ detJe, const double* __restrict__ ws,
-                                                    const double* __restrict__ wf, const double* __restrict__ waveSpe,
-                                                    const double* __restrict__ dAe, double cflfac,
-                                                    double* __restrict__ eleDt, double* part, unsigned* ticket,
-                                                    double* __restrict__ out) {
+__global__ __launch_bounds__(256) void k_ele_dt_em(int E, const double* __restrict__ vole, const double* __restrict__ wf,
+                                                    const double* __restrict__ waveSpe, const double* __restrict__ dAe,
+                                                    double cflfac, double* __restrict__ eleDt, double* part,
+                                                    unsigned* ticket, double* __restrict__ out) {
   using Sh = Shape<D, P>;
-  constexpr int S = Sh::S, F = Sh::F, Ff = Sh::Ff;
+  constexpr int F = Sh::F, Ff = Sh::Ff;
   __shared__ double red[4];
   int e = blockIdx.x * blockDim.x + threadIdx.x;
   double dt = INFINITY;
   if (e < E) {
-    double vol = 0;
-    for (int s = 0; s < S; s++) vol += detJe[(long long)e * S + s] * ws[s];
+    const double vol = vole[e];
     double iw = 0;
     for (int f = 0; f < F; f++) iw += waveSpe[(long long)e * F + f] * wf[f % Ff] * dAe[(long long)e * F + f];
     dt = cflfac * 2 * vol / iw;
     eleDt[e] = dt;
   }
   const double bmin = block_min(dt, red);
+  if (!ticket) return;  // merged meshes: per-mesh minima of eleDt follow
+  if (gridDim.x == 1) {  // one block: its minimum is the global one
+    if (threadIdx.x == 0) *out = bmin;
+    return;
+  }
   if (threadIdx.x == 0) part[blockIdx.x] = bmin;
-  if (!ticket) return;
   __shared__ bool last;
   if (threadIdx.x == 0) {
     // the block minimum above is released with the ticket; the last taker
@@ -2027,7 +2044,7 @@ struct Runner {
       // one mesh: the element kernel's last block forms the global minimum
       Launch L(c, nm("fused_ele_dt", "mixed_ele_dt"));
       const bool one = st.nMesh == 1;
-      hipLaunchKernelGGL((k_ele_dt_em<D, P>), dim3(nb), dim3(256), 0, c.stream, c.z.E, st.detJe, c.ws, c.wf, st.waveSpe,
+      hipLaunchKernelGGL((k_ele_dt_em<D, P>), dim3(nb), dim3(256), 0, c.stream, c.z.E, st.vole, c.wf, st.waveSpe,
                          st.dAe, cflfac, c.eleDt, c.partial, one ? st.dtTicket : nullptr, c.d_dt);
       FCHK(hipGetLastError());
     }
@@ -2144,6 +2161,13 @@ struct Runner {
   static void ensure(Ctx& c) {
     ensure_work<R>(c);
     fit(c);
+    FusedState& st = fs(c);
+    if (!st.vole) {
+      st.vole = dal<double>(c.z.E);
+      hipLaunchKernelGGL((k_ele_vol<D, P>), dim3(nblk(c.z.E, 256)), dim3(256), 0, c.stream, c.z.E, st.detJe, c.ws,
+                         st.vole);
+      FCHK(hipGetLastError());
+    }
   }
 };
 
