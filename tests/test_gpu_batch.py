@@ -102,8 +102,8 @@ def test_batch_pmg_steady_converges():
     b.close()
 
 
-@pytest.mark.parametrize("path", [PATH_FUSED, PATH_MIXED])
-def test_batch_merged_cycle_equals_group_cycles(path):
+@pytest.mark.parametrize("path,low", [(PATH_FUSED, 0), (PATH_MIXED, 0), (PATH_FUSED, 1)])
+def test_batch_merged_cycle_equals_group_cycles(path, low):
     """The merged cycles and one cycle per order group give the same states
     bit for bit, over orders 1, 2 and 4 with two smoothing steps, a gap in the
     orders (no P3 group: the P4 cycle's order-2 level holds P4 meshes only)
@@ -114,10 +114,12 @@ def test_batch_merged_cycle_equals_group_cycles(path):
         ms.append(Member(s, U, ("bump", nx, ny, P, M)))
     out = []
     for merged in (True, False):
-        b = BatchedSolver(ms, path=path, pmg=True, low_order=0, smooth_steps=2, merged_cycle=merged)
+        b = BatchedSolver(ms, path=path, pmg=True, low_order=low, smooth_steps=2, merged_cycle=merged)
         # timeType 6: orders <= 2 and > 2 have their own RK coefficients, so
-        # two merged cycles (P1 and P2; P4)
-        assert [[b.groups[gi].m.order for gi in mc["fine"]] for mc in b._merged] == ([[1, 2], [4]] if merged else [])
+        # two merged cycles (P1 and P2; P4); with lowOrder 1 the P1 group has
+        # no cycle
+        fams = [[1, 2], [4]] if low == 0 else [[2], [4]]
+        assert [[b.groups[gi].m.order for gi in mc["fine"]] for mc in b._merged] == (fams if merged else [])
         r = b.solve_steady(tol=1e-6, max_steps=300, check_every=20)
         out.append((b.states(), r["steps"], r["converged"]))
         b.close()
