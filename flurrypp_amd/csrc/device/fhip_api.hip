@@ -76,9 +76,10 @@ int right_fpt(int D, int P, int Ff, int idr, int relRot, int i) {
 void destroy_ctx(fhip::Ctx& c) {
   if (c.device >= 0) (void)hipSetDevice(c.device);
   fhip::fused_release(c);
-  for (auto& row : c.graph)
-    for (auto& g : row)
-      if (g) (void)hipGraphExecDestroy(g);
+  for (auto& plane : c.graph)
+    for (auto& row : plane)
+      for (auto& g : row)
+        if (g) (void)hipGraphExecDestroy(g);
   void* ptrs[] = {c.sf, c.grad, c.efn, c.corr, c.corru, c.ws, c.wf, c.tp, c.jginv, c.detj, c.norm, c.da,
                   c.ft.ptL, c.ft.ptR, c.ft.bc, c.U, c.U0, c.Uf, c.Fs, c.dU, c.dUf, c.dUc, c.disFn, c.Fn,
                   c.divF, c.waveSp, c.eleDt, c.faceFn, c.src, c.Uacc, c.d_dt, c.partial, c.flag};
@@ -455,7 +456,10 @@ static hipGraphExec_t capture(hipStream_t st, F&& body) {
 
 // One iteration of the main loop body: iter++, solver::update(src).  The
 // coarse P-multigrid levels run update(true) without advancing their counter.
-static void update_step(fhip::Ctx& c, bool fused, bool mixed, bool src = false, bool count = true) {
+// keep = false (fused paths): the step's divF_spts[0] is not stored, for
+// steps that another step or a residual follows before anything reads it.
+static void update_step(fhip::Ctx& c, bool fused, bool mixed, bool src = false, bool count = true,
+                        bool keep = true) {
   const int nRK = c.z.nRK;
   if (count) c.iter++;
   const bool refresh = count && (c.iter == c.prm.init_iter + 1);
@@ -463,11 +467,11 @@ static void update_step(fhip::Ctx& c, bool fused, bool mixed, bool src = false, 
     // Steady-state steps replay one captured graph; the wave-speed refresh
     // step and timed runs launch kernel by kernel.
     if (!refresh && !c.timing && c.use_graphs && !t_capturing_cycle) {
-      hipGraphExec_t& ge = c.graph[src ? 1 : 0][mixed ? 1 + (c.bf16 ? c.bf16 - 1 : 0) : 0];
-      if (!ge) ge = capture(c.stream, [&] { fhip::fused_rk_step(c, false, mixed); });
+      hipGraphExec_t& ge = c.graph[src ? 1 : 0][keep ? 1 : 0][mixed ? 1 + (c.bf16 ? c.bf16 - 1 : 0) : 0];
+      if (!ge) ge = capture(c.stream, [&] { fhip::fused_rk_step(c, false, mixed, keep); });
       HCHK(hipGraphLaunch(ge, c.stream));
     } else {
-      fhip::fused_rk_step(c, refresh, mixed);
+      fhip::fused_rk_step(c, refresh, mixed, keep);
     }
     return;
   }
@@ -496,7 +500,8 @@ int fhip_update(fhip_ctx* h, int n_steps, int path) {
     bool mixed;
     const bool fused = resolve_path(c, path, mixed);
     if (fused && n_steps > 0) fhip::fused_begin(c, mixed);
-    for (int it = 0; it < n_steps; it++) update_step(c, fused, mixed);
+    // divF_spts[0] of the last step only: the earlier ones are overwritten
+    for (int it = 0; it < n_steps; it++) update_step(c, fused, mixed, false, true, it == n_steps - 1);
     if (fused && n_steps > 0) fhip::fused_end(c);
     return 0;
   });
@@ -516,7 +521,7 @@ int fhip_update_many(fhip_ctx** hs, int n, int n_steps, int path) {
     for (int i = 0; i < n; i++)
       if (fused[i]) fhip::fused_begin(hs[i]->c, mixed[i]);
     for (int it = 0; it < n_steps; it++)
-      for (int i = 0; i < n; i++) update_step(hs[i]->c, fused[i], mixed[i]);
+      for (int i = 0; i < n; i++) update_step(hs[i]->c, fused[i], mixed[i], false, true, it == n_steps - 1);
     for (int i = 0; i < n; i++)
       if (fused[i]) fhip::fused_end(hs[i]->c);
     return 0;
@@ -786,7 +791,7 @@ static void pmg_cycle_body_em(fhip_pmg* mg, bool mixed) {
   fhip::fused_em_residual(fine, mixed);
   auto smooth = [&](fhip::Ctx& g) {
     fhip::fused_em_set_src(g, true);
-    for (int k = 0; k < mg->smooth; k++) update_step(g, true, mixed, true, false);
+    for (int k = 0; k < mg->smooth; k++) update_step(g, true, mixed, true, false, false);
     fhip::fused_em_set_src(g, false);
   };
   for (int i = n - 2; i >= 0; i--) {
@@ -830,7 +835,7 @@ static void pmg_cycle_body_batch(fhip_pmg* mg, bool mixed) {
   }
   auto smooth = [&](fhip::Ctx& g) {
     fhip::fused_em_set_src(g, true);
-    for (int k = 0; k < mg->smooth; k++) update_step(g, true, mixed, true, false);
+    for (int k = 0; k < mg->smooth; k++) update_step(g, true, mixed, true, false, false);
     fhip::fused_em_set_src(g, false);
   };
   for (int j = nc - 1; j >= 0; j--) {

@@ -2395,7 +2395,7 @@ void fused_residual(Ctx& c, bool mixed) {
 }
 
 // One iteration of solver::update (src/solver.cpp:319-363) on a fused path.
-void fused_rk_step(Ctx& c, bool refresh, bool mixed) {
+void fused_rk_step(Ctx& c, bool refresh, bool mixed, bool keep) {
   Ops o = get_ops(c, mixed);
   FusedState& st = fs(c);
   const int nRK = c.z.nRK;
@@ -2410,7 +2410,7 @@ void fused_rk_step(Ctx& c, bool refresh, bool mixed) {
       if (refresh) o.wave(c);
       o.dt(c, cflfac);
     }
-    o.rk(c, jameson ? 4 : 3, A[1], B[0], true, 0);
+    o.rk(c, jameson ? 4 : 3, A[1], B[0], keep, 0);
     return;
   }
   if (cfl) {
@@ -2423,7 +2423,7 @@ void fused_rk_step(Ctx& c, bool refresh, bool mixed) {
     int mode;
     if (jameson) mode = step == 0 ? 6 : 4;
     else mode = (step == 0) ? 0 : (step == nRK - 1 ? 2 : 1);
-    o.rk(c, mode, A[step + 1], B[step], step == 0, step);
+    o.rk(c, mode, A[step + 1], B[step], keep && step == 0, step);
   }
 }
 

@@ -84,25 +84,27 @@ struct Ctx {
   bool timing = false;
   Timer timer;
   hipEvent_t ev0 = nullptr, ev1 = nullptr;
-  // captured RK steps (fused path): [with P-multigrid source][FP64, mixed,
-  // mixed with BF16 hi+lo, mixed with BF16 hi+mid+lo].  A graph holds the
-  // device pointers of the moment it was captured: drop them whenever an
-  // array they use is (re)allocated.
-  hipGraphExec_t graph[2][4] = {{nullptr, nullptr, nullptr, nullptr}, {nullptr, nullptr, nullptr, nullptr}};
+  // captured RK steps (fused path): [with P-multigrid source][keeping
+  // divF_spts[0]][FP64, mixed, mixed with BF16 hi+lo, mixed with BF16
+  // hi+mid+lo].  A graph holds the device pointers of the moment it was
+  // captured: drop them whenever an array they use is (re)allocated.
+  hipGraphExec_t graph[2][2][4] = {};
   // mixed path: BF16 pieces of the matrix-core z contractions (0: none)
   int bf16 = 0;
   unsigned long long graph_gen = 0;  // bumped by drop_graphs: graphs over several contexts check it
   void drop_graphs() {
     graph_gen++;
     bool any = false;
-    for (auto& row : graph)
-      for (auto& g : row) any = any || g != nullptr;
+    for (auto& plane : graph)
+      for (auto& row : plane)
+        for (auto& g : row) any = any || g != nullptr;
     if (any) (void)hipStreamSynchronize(stream);  // no replay still in flight
-    for (auto& row : graph)
-      for (auto& g : row) {
-        if (g) (void)hipGraphExecDestroy(g);
-        g = nullptr;
-      }
+    for (auto& plane : graph)
+      for (auto& row : plane)
+        for (auto& g : row) {
+          if (g) (void)hipGraphExecDestroy(g);
+          g = nullptr;
+        }
   }
   bool fused_ok = false;
   bool use_graphs = true;
@@ -165,7 +167,9 @@ void fused_release(Ctx& c);
 // import U, wave speeds [and the P-multigrid source], extrapolate U to the faces
 void fused_begin(Ctx& c, bool mixed, bool src = false);
 void fused_end(Ctx& c);                // export U, divF_spts[0] and wave speeds
-void fused_rk_step(Ctx& c, bool refresh, bool mixed);
+// (keep: write divF_spts[0] at the first stage; steps whose residual nothing
+// reads skip it)
+void fused_rk_step(Ctx& c, bool refresh, bool mixed, bool keep = true);
 // P-multigrid cycle on element-major levels: work arrays and the FAS source
 // allocated; U at the flux points of the current U; the residual (no source)
 // and the wave speeds, the residual written (how) 0: into divF_spts[0],
