@@ -75,8 +75,8 @@ def fused_work(D: int, P: int, E: int, nFF: int, nFFb: int, viscous: bool, nRK: 
         else:
             rd += 2 * E * N * S                      # U0, Uacc
             wr += E * N * S                          # Uacc
-        if st == 0:
-            wr += E * N * S                          # divF_spts[0]
+        # (divF_spts[0] is stored by the last step of an update call only, not
+        # per step: not counted)
         per_stage.append((rd + wr) * B8)
     rk = Work(sum(per_stage) / len(per_stage), E * N * S * 2 * 2 * D)
     if viscous:
@@ -91,7 +91,7 @@ def mixed_work(D: int, P: int, E: int, nFF: int, nFFb: int, viscous: bool, nRK: 
     """Per-launch algorithmic bytes of the mixed-path kernels: FP32 face and
     volume intermediates, fluxes, U at the flux points, stage copy of U and
     stored stage residuals, FP32 copies of the geometry; FP64 state, stage-0
-    copy, divF_spts[0], detJ and wave speeds.  Flops as for the FP64 kernels."""
+    copy, detJ and wave speeds.  Flops as for the FP64 kernels."""
     N, P1 = D + 2, P + 1
     S = P1 ** D
     F = 2 * D * P1 ** (D - 1)
@@ -145,8 +145,6 @@ def mixed_work(D: int, P: int, E: int, nFF: int, nFFb: int, viscous: bool, nRK: 
         else:
             rd_ += E * d8 * N * S                      # U0
             wr_ += E * f4 * N * S                      # residual of this stage
-        if st == 0:
-            wr_ += E * d8 * N * S                      # divF_spts[0]
         per_stage.append(rd_ + wr_)
     rk = sum(per_stage) / len(per_stage)
     if viscous:
