@@ -81,15 +81,16 @@ def footprint_bytes(nDims: int, order: int, nEles: int, viscous: bool = False, n
     SN, FN = S * N, F * N
     # staged: U, U0, F_spts, divF[nRK]; U_fpts, disFn, Fn, face Fn; wave speeds, metrics, face tables
     d = SN * (2 + D + nRK) + 4 * FN + F * (3.5 + D) + D * D * S + S + 1
-    # fused FP64: U, U0, RK accumulator, divF0; wave speeds, metrics (+1/detJ), face tables
-    d += 4 * SN + F * (2 + D) + D * D * S + 2 * S + 1.5 * F
+    # fused FP64: U, U0, RK accumulator, divF0; wave speeds, metrics (+1/detJ), face tables, the
+    # points' side codes, element volumes
+    d += 4 * SN + F * (2 + D) + D * D * S + 2 * S + 1.5 * F + 0.5 * F + 1
     # FP64 work: divF, U/Fn/disFn at the flux points
     d += SN + 3 * FN
     # FP32 work (half a double each): the same, stage residuals, FP32 U, FP32 geometry
     d += 0.5 * (SN + 3 * FN + (nRK - 1) * SN + SN + D * D * S + S + D * F + F)
     if viscous:
         d += D * SN + D * FN + FN            # staged gradients and dUc
-        d += 0.5 * F + D * F                 # fused side codes and face normals
+        d += D * F                           # fused face normals
         d += 2 * FN + 0.5 * (2 * FN + D * F)  # dUc and LDG side parts, FP64 and FP32
     if pmg_source:
         d += SN                              # P-multigrid source, element-major
