@@ -265,6 +265,10 @@ struct FusedState {
   // P-multigrid source term (element-major copy of src_spts), used while use_src
   double* srce = nullptr;
   bool use_src = false;
+  // the stage being formed reads its state from the stage copy Us (the
+  // stages after the first of an RK step); otherwise from the state itself.
+  // FP32 kernels always read the FP32 copy.
+  bool read_stage = false;
   // stream once-touched arrays past the caches: set when U alone (FP64,
   // element-major) exceeds kStreamBytes, i.e. a stage's arrays are many times
   // the 256 MiB infinity cache
@@ -474,8 +478,8 @@ __global__ __launch_bounds__(kMaxBlock) void k_face_inv(int nFF, const int* __re
 // folded into one pass per stage, FP64) and U at the flux points for the
 // next stage.  mode: 0 first classical stage, 1 middle classical stage,
 // 2 last classical stage, 3 single-stage forward Euler, 4 Jameson-form stage,
-// 6 first Jameson-form stage (which also keeps U as the step's U0); 5, 7
-// and 8 the residual alone (see rk_tail_p).
+// 6 first Jameson-form stage (which also keeps U as the step's U0 where a
+// separate U0 is kept); 5, 7 and 8 the residual alone (see rk_tail_p).
 // Classical stages combine into U0 - sum_j b_j r_j / detJ dt in timeStepB's
 // order (src/solver.cpp:504-536): either through the FP64 accumulator Uacc, or
 // (mixed path, Rst given) from the stored stage residuals, which FP32 holds
@@ -578,8 +582,8 @@ __device__ __forceinline__ void rk_tail_p(int E, const R* dv, R* sU, const R* sD
       for (int k = 0; k < N; k++) uin[k] = ld_s<NT>(&Ue[(e * N + k) * S + s]);
     }
     if (needU0) {
-      // with an FP32 stage copy the FP64 state stays the step's U0 until the
-      // last stage writes it, and no separate U0 is kept (U0e null)
+      // with a stage copy the state stays the step's U0 until the last stage
+      // writes it, and no separate U0 is kept (U0e null)
       const double* u0 = U0e ? U0e : Ue;
 #pragma unroll
       for (int k = 0; k < N; k++) u0in[k] = ld_s<NT>(&u0[(e * N + k) * S + s]);
@@ -627,10 +631,13 @@ __device__ __forceinline__ void rk_tail_p(int E, const R* dv, R* sU, const R* sD
         un = u0in[k] - a * r / dj * dt;
       }
       if (frozen) un = u;
-      // with an FP32 stage copy, the FP64 state is written at the step's
-      // last stage only (earlier stages leave it at the step's start)
+      // with a stage copy the state is written at the step's last stage
+      // only (earlier stages leave it at the step's start, its U0, and write
+      // the copy); the FP32 copy is written at the last stage too (the flux
+      // kernels read U only through it), the FP64 one is not (the next
+      // step's first stage reads the state itself)
       if (!Us || fin) st_s<NT>(un, &Ue[iu]);
-      if (Us) st_s<NT>(R(un), &Us[iu]);
+      if (Us && (sizeof(R) == 4 || !fin)) st_s<NT>(R(un), &Us[iu]);
       sU[(b * N + k) * S + spt_lds(D, P1, s)] = R(un);
     }
   }
@@ -1709,7 +1716,7 @@ static void ensure_work(Ctx& c) {
     w.gBnd = dal<R>((long long)D * N * fs(c).nBnd);
   }
   if (sizeof(R) == 4 && c.z.nRK > 1 && c.prm.time_type < 5) w.Rst = dal<R>((long long)(c.z.nRK - 1) * E * N * S);
-  if (sizeof(R) == 4) w.Us = dal<R>(E * N * S);
+  w.Us = dal<R>(E * N * S);
   set_geometry(c, w);
   w.ready = true;
 }
@@ -1734,7 +1741,6 @@ bool fused_setup(Ctx& c, const fhip_desc* d) {
   const long long E = z.E, S = z.S, F = z.F, N = z.N, D = z.D;
   st->stream = E * N * S * (long long)sizeof(double) > kStreamBytes;
   st->Ue = dal<double>(E * N * S);
-  st->U0e = dal<double>(E * N * S);
   st->Uacc = dal<double>(E * N * S);
   st->divF0e = dal<double>(E * N * S);
   st->waveSpe = dal<double>(E * F);
@@ -1963,6 +1969,7 @@ struct Runner {
     FusedState& st = fs(c);
     Work<R>& w = st.work<R>();
     const VFace<R> vf{st.fpart, w.nf, w.Bve, w.gBnd, w.dA, st.nBnd};
+    const R* us = (sizeof(R) == 4 || st.read_stage) ? w.Us : nullptr;
     const Tabs& tb = st.tb;
     // quads and FP32 are held to 4 waves per SIMD (128 VGPRs); FP64 hexes,
     // which spill under that hold, to 3 (168 VGPRs: P3 without spills, where
@@ -1971,18 +1978,18 @@ struct Runner {
       if (c.bf16 == 2 || c.bf16 == 3) {
         auto kern = c.bf16 == 2 ? k_volume_w4<D, P, VISC, R, NT, 2> : k_volume_w4<D, P, VISC, R, NT, 3>;
         hipLaunchKernelGGL(kern, dim3(nblk(c.z.E, nt / S)), dim3(nt), lds(kVolume, nt, VISC), c.stream, c.z.E, st.Ue,
-                           w.Us, w.dUce, w.JG, w.iJ, w.divFe, w.disFne, vf, tb, c.ph, w.Fne, ra);
+                           us, w.dUce, w.JG, w.iJ, w.divFe, w.disFne, vf, tb, c.ph, w.Fne, ra);
         FCHK(hipGetLastError());
         return;
       }
     }
     if constexpr (kHold<D, R>)
       hipLaunchKernelGGL((k_volume_w4<D, P, VISC, R, NT>), dim3(nblk(c.z.E, nt / S)), dim3(nt),
-                         lds(kVolume, nt, VISC), c.stream, c.z.E, st.Ue, w.Us, w.dUce, w.JG, w.iJ, w.divFe, w.disFne,
+                         lds(kVolume, nt, VISC), c.stream, c.z.E, st.Ue, us, w.dUce, w.JG, w.iJ, w.divFe, w.disFne,
                          vf, tb, c.ph, w.Fne, ra);
     else
       hipLaunchKernelGGL((k_volume_w3<D, P, VISC, R, NT>), dim3(nblk(c.z.E, nt / S)), dim3(nt), lds(kVolume, nt, VISC),
-                         c.stream, c.z.E, st.Ue, w.Us, w.dUce, w.JG, w.iJ, w.divFe, w.disFne, vf, tb, c.ph, w.Fne, ra);
+                         c.stream, c.z.E, st.Ue, us, w.dUce, w.JG, w.iJ, w.divFe, w.disFne, vf, tb, c.ph, w.Fne, ra);
     FCHK(hipGetLastError());
   }
   // viscous runs: the volume kernel (ra unused); inviscid runs: the volume
@@ -2418,13 +2425,15 @@ void fused_rk_step(Ctx& c, bool refresh, bool mixed, bool keep) {
     o.dt(c, cflfac);
   }
   for (int step = 0; step < nRK; step++) {
+    st.read_stage = step > 0;
     o.residual(c);
-    // the Jameson form's first stage also stores the step's U0
+    // the Jameson form's first stage reads the state itself (the step's U0)
     int mode;
     if (jameson) mode = step == 0 ? 6 : 4;
     else mode = (step == 0) ? 0 : (step == nRK - 1 ? 2 : 1);
     o.rk(c, mode, A[step + 1], B[step], keep && step == 0, step);
   }
+  st.read_stage = false;
 }
 
 }  // namespace fhip

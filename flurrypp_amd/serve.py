@@ -81,8 +81,8 @@ def footprint_bytes(nDims: int, order: int, nEles: int, viscous: bool = False, n
     SN, FN = S * N, F * N
     # staged: U, U0, F_spts, divF[nRK]; U_fpts, disFn, Fn, face Fn; wave speeds, metrics, face tables
     d = SN * (2 + D + nRK) + 4 * FN + F * (3.5 + D) + D * D * S + S + 1
-    # fused FP64: U, U0, RK accumulator, divF0; wave speeds, metrics (+1/detJ), face tables, the
-    # points' side codes, element volumes
+    # fused FP64: U, its stage copy, RK accumulator, divF0; wave speeds, metrics (+1/detJ), face
+    # tables, the points' side codes, element volumes
     d += 4 * SN + F * (2 + D) + D * D * S + 2 * S + 1.5 * F + 0.5 * F + 1
     # FP64 work: divF, U/Fn/disFn at the flux points
     d += SN + 3 * FN
