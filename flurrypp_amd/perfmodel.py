@@ -68,8 +68,8 @@ def fused_work(D: int, P: int, E: int, nFF: int, nFFb: int, viscous: bool, nRK: 
         elif nRK == 1:
             rd += E * N * S                          # U
         elif st == 0:
-            rd += E * N * S                          # U
-            wr += 2 * E * N * S                      # U0, Uacc
+            rd += E * N * S                          # U (the step's U0: stages before the last write a stage copy)
+            wr += E * N * S                          # Uacc
         elif st == nRK - 1:
             rd += E * N * S                          # Uacc
         else:
