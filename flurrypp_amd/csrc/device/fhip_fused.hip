@@ -1947,8 +1947,10 @@ struct Runner {
   static void launch_extrap(Ctx& c, int nt) {
     FusedState& st = fs(c);
     Work<R>& w = st.work<R>();
+    // the FP32 stage copy starts from the state; the FP64 one is first
+    // written by a step's first stage (which reads the state itself)
     hipLaunchKernelGGL((k_extrap<D, P, R>), dim3(nblk(c.z.E, nt / S)), dim3(nt), lds(kExtrap, nt, false), c.stream,
-                       c.z.E, st.Ue, w.Ufe, w.Us, st.tb);
+                       c.z.E, st.Ue, w.Ufe, sizeof(R) == 4 ? w.Us : nullptr, st.tb);
     FCHK(hipGetLastError());
   }
   static void launch_face_inv(Ctx& c, int nt) {
